@@ -1,0 +1,165 @@
+/* C ABI of libpetite_hip.so: the MI355X (gfx950) shower-transport hot path of
+ * PETITE (kjkellyphys/PETITE).
+ *
+ * The reference has no native boundary (it is pure Python); these entry points
+ * replace the per-particle Python loops listed next to each declaration and
+ * are bound from Python with ctypes (petite_amd/_lib.py).  All array arguments
+ * are DEVICE pointers unless marked "host"; the caller owns every buffer it
+ * passes.  A context owns only its copy of the tables and its scratch queues.
+ * Functions return PETITE_OK (0) or an error code; petite_last_error() gives
+ * the message of the calling thread's last failure.  A context is not
+ * re-entrant: calls on one context must not overlap.
+ */
+#ifndef PETITE_HIP_H
+#define PETITE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PETITE_OK 0
+#define PETITE_ERR_ARG 1
+#define PETITE_ERR_HIP 2
+#define PETITE_ERR_NO_SAMPLE 3 /* draw_sample's "No Sample Found" (src/PETITE/shower.py:460-461) */
+#define PETITE_ERR_CAPACITY 4  /* particle store too small for the shower output */
+
+#define PETITE_NPROC 12 /* Brem Ann PairProd Comp Moller Bhabha MuonE MuonBrem DarkBrem DarkAnn DarkComp DarkMuonBrem */
+
+typedef struct PetiteCtx PetiteCtx;
+typedef void* PetiteStream; /* a hipStream_t (0 = null stream) */
+
+/* VEGAS grid of one process (nodes per E-index: ndim blocks of ninc[d]+1). */
+typedef struct {
+  int32_t n_E;
+  int32_t ndim;
+  int32_t ninc[4];
+  int64_t E_off;    /* offset into the float64 blob: n_E grid energies */
+  int64_t node_off; /* offset into the float32 blob: n_E * stride nodes */
+  int64_t stride;   /* nodes per energy = sum_d (ninc[d] + 1) */
+} PetiteGridDesc;
+
+/* Linear n*sigma table [1/cm], zero outside [E[0], E[n-1]]. */
+typedef struct {
+  int32_t n;
+  int32_t pad;
+  int64_t E_off; /* float64 blob offsets */
+  int64_t v_off;
+} PetiteSigmaDesc;
+
+/* Material, cuts and tables (src/PETITE/shower.py:100-142, 189-246, 273-295). */
+typedef struct {
+  double Z, A, rho, dEdx;  /* dEdx in GeV/m */
+  double mT;               /* target mass put in event_info: A_T (shower.py:435) */
+  double min_energy;       /* tracking cutoff [GeV] */
+  double Eg_min, Ee_min;   /* from the Brem grid parameters (shower.py:214-215) */
+  double rescale_MCS;
+  double maxF_fudge;
+  double min_calc[5];      /* minimum calculable energy for PID 11, -11, 22, 13, -13 */
+  double mV;               /* dark vector mass [GeV] (0 for the SM path) */
+  double Zeff;             /* bound-electron effective charge (dark_shower.py:78) */
+  int64_t max_trials;      /* cap on accept/reject candidates (max_n_integrators * neval) */
+  int32_t MS;              /* multiple scattering on charged particles (generate_shower GlobalMS) */
+  int32_t pad;
+  PetiteGridDesc grid[PETITE_NPROC];
+  PetiteSigmaDesc sigma[PETITE_NPROC];
+} PetiteTableDesc;
+
+/* One primary particle (generate_shower's p0; src/PETITE/particle.py:53-95). */
+typedef struct {
+  double E, px, py, pz; /* four-momentum [GeV] */
+  double x, y, z;       /* start position [m] */
+  double mass;          /* particle mass [GeV] */
+  double weight;
+  int32_t pid;
+  int32_t shower;       /* shower index written to the output records */
+  uint64_t sid;         /* Philox stream id of the primary */
+} PetitePrimary;
+
+/* Device particle store (structure of arrays, capacity entries each).  One
+ * entry per output Particle: p0 (E,px,py,pz), r0 (x,y,z,weight), pf, rf
+ * (x,y,z,steps), pid, parent slot (-1 for primaries), meta (generation |
+ * process+1 << 16 | child index << 24), shower index, stream id. */
+typedef struct {
+  float* p0;
+  float* r0;
+  float* pf;
+  float* rf;
+  int32_t* pid;
+  int32_t* parent;
+  int32_t* meta;
+  int32_t* shower;
+  uint64_t* sid;
+  int64_t capacity;
+} PetiteStore;
+
+/* Counters of one run (host struct, filled on return). */
+typedef struct {
+  int64_t n_particles;  /* records written */
+  int64_t n_steps;      /* particle-steps (charged substeps + closing increments + photon flights) */
+  int64_t n_draws;      /* accept/reject draws */
+  int64_t n_trials;     /* candidates evaluated */
+  int64_t n_maxf_viol;  /* accepted candidates with J*f > max_F */
+  int64_t n_no_sample;  /* draws that hit max_trials */
+  int64_t n_iterations; /* generation sweeps */
+  int64_t n_dropped;    /* children below min_energy */
+  /* kernel times measured with HIP events on the run's stream (0 unless
+   * timing is enabled with petite_set_timing) */
+  double ms_transport, ms_draw, ms_emit;
+  int64_t launches_transport, launches_draw, launches_emit;
+} PetiteStats;
+
+/* Create a context: copies the table blobs (host pointers) to the device. */
+int petite_ctx_create(const PetiteTableDesc* desc, const double* blob64, int64_t n64, const float* blob32,
+                      int64_t n32, PetiteCtx** out);
+void petite_ctx_destroy(PetiteCtx* ctx);
+const char* petite_last_error(void);
+/* Enable (1) or disable (0) per-kernel HIP-event timing in petite_run_showers.
+ * No counterpart in the reference: it feeds the roofline figures. */
+int petite_set_timing(PetiteCtx* ctx, int32_t on);
+
+/* max_F table of one process for the context material: the largest J*f over
+ * n_samples map candidates at each grid energy (utilities/find_maxes.py:55-119).
+ * Writes the device copy used by the draws; maxf_host/mean_host (n_E doubles,
+ * may be NULL) receive the maxima and the sigma estimates (mean J*f). */
+int petite_maxf_scan(PetiteCtx* ctx, int32_t process, int32_t n_samples, uint64_t seed, double* maxf_host,
+                     double* mean_host, PetiteStream stream);
+/* Replace the max_F table of one process (host array of n_E doubles). */
+int petite_set_maxf(PetiteCtx* ctx, int32_t process, const double* maxf_host, int32_t n_E);
+
+/* Batched accept/reject draws: replaces Shower.draw_sample
+ * (src/PETITE/shower.py:401-465) and DarkShower.draw_dark_sample
+ * (src/PETITE/dark_shower.py:649-704).  Request i draws process proc[i] at
+ * energy E[i] from the Philox stream sid[i]; x_out[4i..4i+3] receives the
+ * accepted point, trials_out[i] the candidate count (the reference's VB
+ * count) and w_out[i] its J*f.  A request with no accepted candidate within
+ * max_trials gets trials_out = -max_trials (0 for a process without a grid)
+ * and makes the call return PETITE_ERR_NO_SAMPLE. */
+int petite_vegas_draw(PetiteCtx* ctx, const int32_t* proc, const double* E, const uint64_t* sid, int64_t n,
+                      uint64_t seed, float* x_out, int32_t* trials_out, float* w_out, PetiteStream stream);
+
+/* J*f of single map candidates: for request i, the map of proc[i] at
+ * LU_Key(E[i]) applied to the four Philox words words[4i..4i+3] (the same
+ * words a draw consumes), and the integrand at the true energy E[i].  The
+ * per-candidate quantity compared in draw_sample's accept test
+ * (src/PETITE/shower.py:453-457).  x_out: n x 4 doubles, w_out: n doubles. */
+int petite_eval_points(PetiteCtx* ctx, const int32_t* proc, const double* E, const uint32_t* words, int64_t n,
+                       double* x_out, double* w_out, PetiteStream stream);
+
+/* Whole showers: replaces Shower.generate_shower's particle loop
+ * (src/PETITE/shower.py:603-708) with propagate_particle (509-601),
+ * sample_scattering (467-507) and draw_sample (401-465) for every particle.
+ * prim: device array of n primaries.  The store receives every output
+ * particle (primaries first, then children by generation).  shower_energy
+ * (device, 4 doubles per shower, may be NULL) accumulates deposited dE/dx,
+ * final energy of particles that end, energy of dropped children and the
+ * number of atomic-electron targets (in units of m_e). */
+int petite_run_showers(PetiteCtx* ctx, const PetitePrimary* prim, int64_t n, uint64_t seed, PetiteStore* store,
+                       double* shower_energy, PetiteStats* stats, PetiteStream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PETITE_HIP_H */

@@ -1,0 +1,107 @@
+"""ctypes binding of libpetite_hip.so (declarations: include/petite_hip.h).
+
+The library is loaded from this package directory only; there is no CPU
+fallback.  ``load()`` raises if the shared object is missing or if no HIP
+device is present when a compute entry point is used.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpetite_hip.so")
+
+NPROC = 12
+PETITE_OK = 0
+PETITE_ERR_ARG = 1
+PETITE_ERR_HIP = 2
+PETITE_ERR_NO_SAMPLE = 3
+PETITE_ERR_CAPACITY = 4
+
+EXPORTED = ["petite_ctx_create", "petite_ctx_destroy", "petite_last_error", "petite_maxf_scan", "petite_set_maxf",
+            "petite_vegas_draw", "petite_eval_points", "petite_run_showers", "petite_set_timing"]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("n_E", C.c_int32), ("ndim", C.c_int32), ("ninc", C.c_int32 * 4), ("E_off", C.c_int64),
+                ("node_off", C.c_int64), ("stride", C.c_int64)]
+
+
+class SigmaDesc(C.Structure):
+    _fields_ = [("n", C.c_int32), ("pad", C.c_int32), ("E_off", C.c_int64), ("v_off", C.c_int64)]
+
+
+class TableDesc(C.Structure):
+    _fields_ = [("Z", C.c_double), ("A", C.c_double), ("rho", C.c_double), ("dEdx", C.c_double),
+                ("mT", C.c_double), ("min_energy", C.c_double), ("Eg_min", C.c_double), ("Ee_min", C.c_double),
+                ("rescale_MCS", C.c_double), ("maxF_fudge", C.c_double), ("min_calc", C.c_double * 5),
+                ("mV", C.c_double), ("Zeff", C.c_double), ("max_trials", C.c_int64), ("MS", C.c_int32),
+                ("pad", C.c_int32), ("grid", GridDesc * NPROC), ("sigma", SigmaDesc * NPROC)]
+
+
+class Primary(C.Structure):
+    _fields_ = [("E", C.c_double), ("px", C.c_double), ("py", C.c_double), ("pz", C.c_double),
+                ("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("mass", C.c_double),
+                ("weight", C.c_double), ("pid", C.c_int32), ("shower", C.c_int32), ("sid", C.c_uint64)]
+
+
+class Store(C.Structure):
+    _fields_ = [("p0", C.c_void_p), ("r0", C.c_void_p), ("pf", C.c_void_p), ("rf", C.c_void_p),
+                ("pid", C.c_void_p), ("parent", C.c_void_p), ("meta", C.c_void_p), ("shower", C.c_void_p),
+                ("sid", C.c_void_p), ("capacity", C.c_int64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_particles", C.c_int64), ("n_steps", C.c_int64), ("n_draws", C.c_int64),
+                ("n_trials", C.c_int64), ("n_maxf_viol", C.c_int64), ("n_no_sample", C.c_int64),
+                ("n_iterations", C.c_int64), ("n_dropped", C.c_int64), ("ms_transport", C.c_double),
+                ("ms_draw", C.c_double), ("ms_emit", C.c_double), ("launches_transport", C.c_int64),
+                ("launches_draw", C.c_int64), ("launches_emit", C.c_int64)]
+
+    def as_dict(self):
+        return {f: (float(getattr(self, f)) if t is C.c_double else int(getattr(self, f))) for f, t in self._fields_}
+
+
+class PetiteError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[petite {code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load the in-tree libpetite_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m petite_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    vp, i32, i64, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64
+    lib.petite_ctx_create.argtypes = [C.POINTER(TableDesc), vp, i64, vp, i64, C.POINTER(vp)]
+    lib.petite_ctx_create.restype = C.c_int
+    lib.petite_ctx_destroy.argtypes = [vp]
+    lib.petite_ctx_destroy.restype = None
+    lib.petite_last_error.argtypes = []
+    lib.petite_last_error.restype = C.c_char_p
+    lib.petite_set_timing.argtypes = [vp, i32]
+    lib.petite_set_timing.restype = C.c_int
+    lib.petite_maxf_scan.argtypes = [vp, i32, i32, u64, vp, vp, vp]
+    lib.petite_maxf_scan.restype = C.c_int
+    lib.petite_set_maxf.argtypes = [vp, i32, vp, i32]
+    lib.petite_set_maxf.restype = C.c_int
+    lib.petite_vegas_draw.argtypes = [vp, vp, vp, vp, i64, u64, vp, vp, vp, vp]
+    lib.petite_vegas_draw.restype = C.c_int
+    lib.petite_eval_points.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp]
+    lib.petite_eval_points.restype = C.c_int
+    lib.petite_run_showers.argtypes = [vp, vp, i64, u64, C.POINTER(Store), vp, C.POINTER(Stats), vp]
+    lib.petite_run_showers.restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def check(rc, what=""):
+    if rc != PETITE_OK:
+        msg = load().petite_last_error().decode(errors="replace")
+        raise PetiteError(rc, f"{what}: {msg}" if what else msg)

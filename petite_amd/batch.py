@@ -1,0 +1,120 @@
+"""Device-resident shower output and its conversion to ``Particle`` lists.
+
+A ``ParticleBatch`` wraps the particle store filled by petite_run_showers: one
+record per output particle of the reference's generate_shower
+(src/PETITE/shower.py:603-708).  ``to_particles()`` rebuilds Particle objects
+with the reference's ids dictionary, including the arbitrary-precision
+``ID = 2 * parent_ID + k`` (shower.py:486,494) reconstructed from the parent
+links (the device keeps parent slots, not the doubling IDs, which exceed 64
+bits in deep showers).
+"""
+import numpy as np
+import torch
+
+from .constants import mass_dict, process_name
+from .particle import Particle
+
+PRIMARY_DTYPE = np.dtype([("E", "f8"), ("px", "f8"), ("py", "f8"), ("pz", "f8"), ("x", "f8"), ("y", "f8"),
+                          ("z", "f8"), ("mass", "f8"), ("weight", "f8"), ("pid", "i4"), ("shower", "i4"),
+                          ("sid", "u8")])
+
+
+class ParticleBatch:
+    def __init__(self, store, n, stats=None, energy=None, primaries=None):
+        self.n = int(n)
+        self._store = store
+        self.stats = stats or {}
+        self.energy = energy
+        self.primaries = primaries
+
+    def __len__(self):
+        return self.n
+
+    def tensor(self, key):
+        return self._store[key][: self.n]
+
+    @property
+    def p0(self):
+        return self.tensor("p0")
+
+    @property
+    def pf(self):
+        return self.tensor("pf")
+
+    @property
+    def r0(self):
+        return self.tensor("r0")
+
+    @property
+    def rf(self):
+        return self.tensor("rf")
+
+    @property
+    def pid(self):
+        return self.tensor("pid")
+
+    @property
+    def parent(self):
+        return self.tensor("parent")
+
+    @property
+    def shower(self):
+        return self.tensor("shower")
+
+    @property
+    def generation(self):
+        return self.tensor("meta") & 0xFFFF
+
+    @property
+    def process(self):
+        """Process code + 1 of the vertex that made the particle (0 = primary)."""
+        return (self.tensor("meta") >> 16) & 0xFF
+
+    @property
+    def child_index(self):
+        return (self.tensor("meta") >> 24) & 0x1
+
+    @property
+    def sid(self):
+        return self.tensor("sid")
+
+    def to_numpy(self):
+        out = {k: self._store[k][: self.n].cpu().numpy() for k in self._store}
+        out["sid"] = out["sid"].view(np.uint64)
+        return out
+
+    def to_particles(self, shower=None):
+        """List of Particle objects (all showers, or one), primary first per shower."""
+        a = self.to_numpy()
+        idx = np.arange(self.n)
+        if shower is not None:
+            idx = idx[a["shower"] == shower]
+        ids = {}
+        out = []
+        prim_w = {}
+        for i in idx:
+            par = int(a["parent"][i])
+            meta = int(a["meta"][i])
+            gen = meta & 0xFFFF
+            proc = (meta >> 16) & 0xFF
+            k = (meta >> 24) & 1
+            pid = int(a["pid"][i])
+            if par < 0:
+                ID, parent_ID, parent_PID = 1, -1, 22
+                process = "Input"
+            else:
+                ID = 2 * ids[par] + k
+                parent_ID = ids[par]
+                parent_PID = int(a["pid"][par])
+                process = process_name[proc - 1]
+            ids[i] = ID
+            d = {"PID": pid, "ID": ID, "parent_PID": parent_PID, "parent_ID": parent_ID, "generation_number": gen,
+                 "generation_process": process, "weight": float(a["r0"][i][3]), "mass": mass_dict.get(pid, 0.0)}
+            if par < 0 and self.primaries is not None:
+                d["mass"] = float(self.primaries["mass"][i]) if i < len(self.primaries) else d["mass"]
+            p = Particle(a["p0"][i].astype(np.float64), a["r0"][i][:3].astype(np.float64), d)
+            p.set_pf(a["pf"][i].astype(np.float64))
+            p.set_rf(a["rf"][i][:3].astype(np.float64))
+            p.set_ended(True)
+            out.append(p)
+        return out

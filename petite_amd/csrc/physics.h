@@ -1,0 +1,387 @@
+// Device physics of the PETITE shower-transport path for gfx950.
+//
+// Cross sections follow src/PETITE/all_processes.py of the reference, with
+// the 4-D integrands (Brem, MuonBrem, PairProd) evaluated in float32 after a
+// float64 prologue that forms the cancellation-prone variables (1-x, x2-x3,
+// ep-w) exactly, and with PF*FF folded so that no q^4/q^4 ratio is formed in
+// float32.  The 1-D and dark integrands run in float64 with the reference's
+// own formulas.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "petite_hip.h"
+#include "philox.h"
+
+namespace petite {
+
+constexpr double kAlpha = 1.0 / 137.035999;
+constexpr double kMe = 510.998950e-6;
+constexpr double kMmu = 105.6583755e-3;
+constexpr double kMp = 938.272088e-3;
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kCmToM = 0.01;
+
+enum Proc : int32_t {
+  BREM = 0, ANN = 1, PAIRPROD = 2, COMP = 3, MOLLER = 4, BHABHA = 5, MUONE = 6, MUONBREM = 7,
+  DARKBREM = 8, DARKANN = 9, DARKCOMP = 10, DARKMUONBREM = 11
+};
+
+struct DevTables {
+  const double* f64;
+  const float* f32;
+  const double* maxf;  // [PETITE_NPROC][maxE]
+  int32_t maxE;
+  float brem_C_e;      // 8/pi alpha (alpha/m)^2 m^4 Z^2 a0^4 for m = m_e
+  float brem_C_mu;     // same for m = m_mu (a0 keeps m_e, all_processes.py:92-103)
+  float a0sq;          // a0^2 [GeV^-2]
+  float pad;
+  PetiteTableDesc d;
+};
+
+// ---------------------------------------------------------------- tables ---
+
+// Linear interpolation with zero outside [E0, E_{n-1}] (scipy interp1d with
+// fill_value=0, bounds_error=False as in src/PETITE/shower.py:280-295).
+__device__ inline double nsigma(const DevTables& T, int proc, double E) {
+  const PetiteSigmaDesc& s = T.d.sigma[proc];
+  if (s.n <= 0) return 0.0;
+  const double* xe = T.f64 + s.E_off;
+  const double* ye = T.f64 + s.v_off;
+  if (!(E >= xe[0]) || E > xe[s.n - 1]) return 0.0;
+  int lo = 0, hi = s.n - 1;  // xe[lo] <= E <= xe[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (xe[mid] <= E) lo = mid; else hi = mid;
+  }
+  const double t = (E - xe[lo]) / (xe[hi] - xe[lo]);
+  return ye[lo] + t * (ye[hi] - ye[lo]);
+}
+
+// Channels per species in the reference's order (shower.py:665-698).
+__device__ inline int channels(int pid, int* ch) {
+  switch (pid) {
+    case 11: ch[0] = BREM; ch[1] = MOLLER; return 2;
+    case -11: ch[0] = BREM; ch[1] = ANN; ch[2] = BHABHA; return 3;
+    case 22: ch[0] = PAIRPROD; ch[1] = COMP; return 2;
+    case 13: case -13: ch[0] = MUONE; ch[1] = MUONBREM; return 2;
+    default: return 0;
+  }
+}
+
+// get_mfp (shower.py:370-389): metres, 1e12 when n*sigma <= 0.
+__device__ inline double mean_free_path(const DevTables& T, int pid, double E) {
+  int ch[3];
+  const int n = channels(pid, ch);
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += nsigma(T, ch[i], E);
+  return s > 0.0 ? kCmToM / s : 1.0e12;
+}
+
+// LU_Key = argmin|E_i - E| + 1, clamped to n_E - 1 (shower.py:416-426).
+__device__ inline int lu_key(const DevTables& T, int proc, double E) {
+  const PetiteGridDesc& g = T.d.grid[proc];
+  const double* Eg = T.f64 + g.E_off;
+  int lo = 0, hi = g.n_E - 1;
+  if (E <= Eg[0]) return g.n_E > 1 ? 1 : 0;
+  if (E >= Eg[hi]) return hi;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (Eg[mid] <= E) lo = mid; else hi = mid;
+  }
+  // argmin picks the first index on ties (numpy.argmin)
+  int best = (fabs(Eg[lo] - E) <= fabs(Eg[hi] - E)) ? lo : hi;
+  int k = best + 1;
+  return k > g.n_E - 1 ? g.n_E - 1 : k;
+}
+
+// VEGAS map transform with an integer-exact increment index: the uniform
+// y = (2m+1) 2^-24 (m = the top 23 bits of the Philox word) times N is formed
+// as an exact 34-bit integer, so the bin and the fraction equal what float64
+// gives on the host.
+__device__ inline double map_point(const DevTables& T, int proc, int ie, const uint32_t* r, int ndim, double* x) {
+  const PetiteGridDesc& g = T.d.grid[proc];
+  const float* base = T.f32 + g.node_off + (int64_t)ie * g.stride;
+  double J = 1.0;
+  for (int d = 0; d < ndim; ++d) {
+    const uint32_t N = (uint32_t)g.ninc[d];
+    const uint64_t q = (uint64_t)(2u * (r[d] >> 9) + 1u) * (uint64_t)N;
+    const uint32_t i = (uint32_t)(q >> 24);
+    const double fr = (double)(uint32_t)(q & 0xFFFFFFu) * 0x1.0p-24;
+    const double g0 = (double)base[i];
+    const double dg = (double)base[i + 1] - g0;
+    x[d] = g0 + dg * fr;
+    J *= (double)N * dg;
+    base += N + 1;
+  }
+  return J;
+}
+
+// -------------------------------------------------------------- integrands --
+
+// sin(pi t) and cos(pi t) in float32 with the argument reduced in float64
+// first (t - rint(t) is exact), so the float32 sine sees |arg| <= pi/2 with
+// full relative precision near the zeros.
+__device__ inline float sinpi_f(double t) {
+  const double n = rint(t);
+  const float s = sinf((float)((t - n) * kPi));
+  return ((int64_t)n & 1) ? -s : s;
+}
+__device__ inline float cospi_f(double t) {
+  // cos(pi t) = sin(pi (1/2 - t))
+  return sinpi_f(0.5 - t);
+}
+
+// Brem / MuonBrem, all_processes.py:139-206.  PF*FF = C epp d dp /(w ep (1+a0^2 q^2)^2).
+__device__ inline float f_brem(const DevTables& T, double ep, double m, float C, const double* x) {
+  const double egmin = T.d.Eg_min;
+  const double range = ep - m - egmin;
+  const double w = egmin + x[0] * range;
+  const double epp = ep - w;
+  const double k = ep / (2.0 * m);
+  const double d = k * (x[1] + x[2]);
+  const double dp = k * (x[1] - x[2]);
+  if (!((egmin < w) && (w < ep - m) && (m < epp) && (epp < ep) && (d > 0.0) && (dp > 0.0))) return 0.0f;
+  const float fd = (float)d, fdp = (float)dp, fw = (float)w, fepp = (float)epp, fep = (float)ep, fm = (float)m;
+  const float dmd = (float)(ep / m * x[2]);                  // d - dp
+  const float Bn = (float)(-w + d * d * epp - dp * dp * ep);  // (1+d^2) epp - (1+dp^2) ep
+  const float s2 = sinpi_f(x[3] - 0.5);                      // sin(ph/2)
+  const float B = Bn / (2.0f * fep * fepp);
+  const float qm2 = dmd * dmd + 4.0f * fd * fdp * s2 * s2 + fm * fm * B * B;
+  const float qsq = fm * fm * qm2;
+  const float den = 1.0f + T.a0sq * qsq;
+  const float pfff = C * fepp * fd * fdp / (fw * fep * den * den);
+  const float jac = (float)kPi * fep * fep * (float)range / (fm * fm);
+  // T1+T2+T3+T4 of the reference as a sum of non-negative terms: the direct
+  // sum cancels to O((w/ep)^2) for soft photons.  With a=d, b=dp, A=1/(1+a^2),
+  // B=1/(1+b^2):  (a-b)^2 (1-ab)^2 A^2 B^2 + 4 a b sin^2(ph/2) A B
+  //               + w^2/(2 ep epp) A B ((a-b)^2 + 4 a b sin^2(ph/2))
+  const float i1 = 1.0f / (1.0f + fd * fd), i2 = 1.0f / (1.0f + fdp * fdp);
+  const float ab = fd * fdp;
+  const float q1 = dmd * (1.0f - ab) * i1 * i2;
+  const float ang = 4.0f * ab * s2 * s2;
+  const float bracket = q1 * q1 + ang * i1 * i2 + fw * fw / (2.0f * fep * fepp) * i1 * i2 * (dmd * dmd + ang);
+  return pfff * bracket * jac;
+}
+
+// PairProd, all_processes.py:531-622.
+__device__ inline float f_pairprod(const DevTables& T, double w, const double* x) {
+  const double m = kMe;
+  const double r2 = w - 2.0 * m;
+  const double epp = m + x[0] * r2;
+  const double epm = m + (1.0 - x[0]) * r2;
+  const double k = w / (2.0 * m);
+  const double dp = k * (x[1] + x[2]);
+  const double dm = k * (x[1] - x[2]);
+  if (!((m < epm) && (epm < w) && (m < epp) && (epp < w) && (dm > 0.0) && (dp > 0.0))) return 0.0f;
+  const float fdp = (float)dp, fdm = (float)dm, fw = (float)w, fepp = (float)epp, fepm = (float)epm, fm = (float)m;
+  const float dpm = (float)(w / m * x[2]);                   // dp - dm
+  const float c2 = cospi_f(x[3]);                            // cos(ph/2)
+  const float dp2 = fdp * fdp, dm2 = fdm * fdm;
+  const float S = (1.0f + dp2) / (2.0f * fepp) + (1.0f + dm2) / (2.0f * fepm);
+  const float qq = dpm * dpm + 4.0f * fdp * fdm * c2 * c2 + fm * fm * S * S;
+  const float den = 1.0f + T.a0sq * fm * fm * qq;
+  const float pfff = T.brem_C_e * fepp * fepm * fdp * fdm / (fw * fw * fw * den * den);
+  const float jac = (float)kPi * fw * fw * (float)r2 / (fm * fm);
+  // T1+T2+T3+T4 of the reference as a sum of non-negative terms (a=dp, b=dm):
+  //   (a^2-b^2)^2 A^2 B^2 + (epp^2+epm^2)/(2 epp epm) A B ((a-b)^2 + 4 a b cos^2(ph/2))
+  const float i1 = 1.0f / (1.0f + dp2), i2 = 1.0f / (1.0f + dm2);
+  const float t1 = dpm * (fdp + fdm) * i1 * i2;
+  const float bracket = t1 * t1 + (fepp * fepp + fepm * fepm) / (2.0f * fepp * fepm) * i1 * i2 *
+                                      (dpm * dpm + 4.0f * fdp * fdm * c2 * c2);
+  return pfff * bracket * jac;
+}
+
+// Compton / DarkComp on a free electron at rest, all_processes.py:624-742.
+__device__ inline double f_compton(double Eg, double mV, double ct) {
+  const double m = kMe;
+  const double m2 = m * m, mV2 = mV * mV;
+  const double s = m2 + 2.0 * Eg * m;
+  if (s < (m + mV) * (m + mV)) return 0.0;
+  const double R = sqrt(m2 * m2 + (mV2 - s) * (mV2 - s) - 2.0 * m2 * (mV2 + s));
+  const double jac = (s - m2) / (2.0 * s) * sqrt((s - mV2) * (s - mV2) - 2.0 * m2 * (s + mV2) + m2 * m2);
+  const double t = -0.5 * (m2 * m2 + s * (-mV2 + s + ct * R) - m2 * (mV2 + 2.0 * s + ct * R)) / s;
+  const double PF = 2.0 * kPi * kAlpha * kAlpha / ((s - m2) * (s - m2));
+  const double T1 = (2.0 * m2 * (mV2 - 3.0 * s) - 3.0 * m2 * m2 - 2.0 * mV2 * s + 2.0 * mV2 * mV2 + s * s) /
+                    ((m2 - s) * (m2 + mV2 - s - t));
+  const double D = m2 + mV2 - s - t;
+  const double T2 = (2.0 * m2 * (2.0 * m2 + mV2)) / (D * D);
+  const double T3 = ((m2 + s) * (m2 + mV2 + s) + t * (s - m2)) / ((m2 - s) * (m2 - s));
+  return PF * jac * (T1 + T2 + T3);
+}
+
+// Ann (e+ e- -> gamma gamma), all_processes.py:468-529 (mV = 0, Eg_min cut).
+__device__ inline double f_annihilation(double Ee, double mV, double egmin, double ct) {
+  const double m = kMe;
+  const double s = 2.0 * m * (Ee + m);
+  const double ctMax = sqrt((Ee + m) / (Ee - m)) * (2.0 * m * (Ee - 2.0 * egmin + m) - mV * mV) /
+                       (2.0 * m * (Ee + m) - mV * mV);
+  if (s < mV * mV) return 0.0;
+  if (ct > ctMax) return 0.0;
+  const double b2 = 1.0 - 4.0 * m * m / s;
+  return 4.0 * kPi * kAlpha * kAlpha / (s * (1.0 - b2 * ct * ct)) *
+         ((s - mV * mV) / (2.0 * s) * (1.0 + ct * ct) + 2.0 * mV * mV / (s - mV * mV));
+}
+
+// Moller, all_processes.py:786-840.
+__device__ inline double f_moller(double Ee, double DE, double ct) {
+  const double m = kMe;
+  const double dct = 2.0 * DE / (Ee - m);
+  if (!((ct > -1.0 + dct) && (ct < 1.0 - dct))) return 0.0;
+  const double s = m * m + 2.0 * Ee * m;
+  const double c2 = ct * ct;
+  const double num = s * s * (3.0 + c2) * (3.0 + c2) - 8.0 * m * m * s * (7.0 + c2 * c2) +
+                     16.0 * m * m * m * m * (6.0 - 3.0 * c2 + c2 * c2);
+  const double sm = s - 4.0 * m * m;
+  return 16.0 * kPi * kPi * kAlpha * kAlpha * num /
+         (8.0 * kPi * s * sm * sm * (1.0 - ct) * (1.0 - ct) * (1.0 + ct) * (1.0 + ct));
+}
+
+// Bhabha, all_processes.py:947-1007.
+__device__ inline double f_bhabha(double Ee, double DE, double ct) {
+  const double m = kMe;
+  const double dct = 2.0 * DE / (Ee - m);
+  if (!((ct > -1.0 + dct) && (ct < 1.0 - dct))) return 0.0;
+  const double s = m * m + 2.0 * Ee * m;
+  const double m2 = m * m, m4 = m2 * m2, m6 = m4 * m2, m8 = m4 * m4;
+  const double cm1 = -1.0 + ct;
+  const double num = 256.0 * cm1 * cm1 * ct * ct * m8 -
+                     128.0 * cm1 * (1.0 + ct * (1.0 + ct) * (-3.0 + 2.0 * ct)) * m6 * s +
+                     16.0 * (7.0 + ct * (2.0 + ct * (-5.0 + 6.0 * cm1 * ct))) * m4 * s * s -
+                     8.0 * (7.0 + ct * (-3.0 + ct * (3.0 + ct * (-1.0 + 2.0 * ct)))) * m2 * s * s * s +
+                     (3.0 + ct * ct) * (3.0 + ct * ct) * s * s * s * s;
+  const double sm = -4.0 * m2 + s;
+  return kAlpha * kAlpha * kPi * num / (2.0 * cm1 * cm1 * s * s * s * sm * sm);
+}
+
+// mu e -> mu e, all_processes.py:842-886.
+__device__ inline double f_muone(double Emu, double DE, double ct) {
+  const double me = kMe, mm = kMmu;
+  const double s = me * me + mm * mm + 2.0 * me * Emu;
+  const double t_limit = 2.0 * me * (me - DE);
+  const double a = s + me * me - mm * mm;
+  const double t = -2.0 * (1.0 - ct) * (a * a / (4.0 * s) - me * me);
+  if (!(t < t_limit)) return 0.0;
+  const double b = s + t - 4.0 * me * me;
+  return 16.0 * kPi * kPi * kAlpha * kAlpha *
+         (s * s + 2.0 * (me * me + mm * mm) * (2.0 * t + mm * mm - 3.0 * me * me) + b * b) / (16.0 * kPi * s * t * t);
+}
+
+// Gninenko et al. elastic+inelastic form factor over t^2, all_processes.py:111-133.
+__device__ inline double g_el_inel_over_t2(double Z, double A, double t) {
+  const double mu_p = 2.79;
+  const double c1 = pow(111.0 * pow(Z, -1.0 / 3.0) / kMe, 2.0);
+  const double c2 = 0.164 * pow(A, -2.0 / 3.0);
+  const double a1 = 1.0 / (1.0 + c1 * t);
+  const double a2 = 1.0 + t / c2;
+  const double Gel = a1 * a1 / (a2 * a2);
+  const double ap2 = pow(773.0 * pow(Z, -2.0 / 3.0) / kMe, 2.0);
+  const double b1 = ap2 / (1.0 + ap2 * t);
+  const double b2 = 1.0 + t / 0.71;
+  const double Ginel = Z / (c1 * c1 * Z * Z) * b1 * b1 *
+                       ((1.0 + (mu_p * mu_p - 1.0) * t / (4.0 * kMp * kMp)) / (b2 * b2 * b2 * b2));
+  return Z * Z * c1 * c1 * (Gel + Ginel);
+}
+
+// DarkBrem / DarkMuonBrem exact tree level, "Log" variables, all_processes.py:208-372.
+__device__ inline double f_dark_brem(double Ebeam, double m_lepton, double mV, double MT, double Z, double A,
+                                     const double* xv) {
+  const double x = xv[0];
+  const double one_minus_ct = pow(10.0, xv[1]);
+  const double ct = 1.0 - one_minus_ct;
+  const double ttilde = pow(10.0, xv[2]);
+  const double ln10 = 2.302585092994046;
+  const double Jac = one_minus_ct * ttilde * ln10 * ln10;
+  const double k = sqrt(fabs((x * Ebeam) * (x * Ebeam) - mV * mV));
+  const double p = sqrt(Ebeam * Ebeam - m_lepton * m_lepton);
+  const double V = sqrt(p * p + k * k - 2.0 * p * k * ct);
+  const double ut = -2.0 * (x * Ebeam * Ebeam - k * p * ct) + mV * mV;
+  const double E1 = (1.0 - x) * Ebeam + MT;
+  const double discr = ut * ut + 4.0 * MT * ut * E1 + 4.0 * MT * MT * V * V;
+  const double sq = sqrt(fabs(discr));
+  const double den = 2.0 * E1 * E1 - 2.0 * V * V;
+  const double Qp = fabs((V * (ut + 2.0 * MT * E1) + E1 * sq) / den);
+  const double Qm = fabs((V * (ut + 2.0 * MT * E1) - E1 * sq) / den);
+  const double tplus = 2.0 * MT * (sqrt(MT * MT + Qp * Qp) - MT);
+  const double tminus = 2.0 * MT * (sqrt(MT * MT + Qm * Qm) - MT);
+  const double tc = 2.0 * MT * (MT + Ebeam) * sqrt(Ebeam * Ebeam + m_lepton * m_lepton) /
+                    (MT * (MT + 2.0 * Ebeam) + m_lepton * m_lepton);
+  const double tconv = tc * tc;
+  const double t = ttilde * tconv;
+  const double q0 = -t / (2.0 * MT);
+  const double q = sqrt(t * t / (4.0 * MT * MT) + t);
+  const double cq = -(V * V + q * q + m_lepton * m_lepton - (Ebeam + q0 - x * Ebeam) * (Ebeam + q0 - x * Ebeam)) /
+                    (2.0 * V * q);
+  const double mv2 = mV * mV + 2.0 * m_lepton * m_lepton;
+  const double Am2 = -8.0 * MT * (4.0 * Ebeam * Ebeam * MT - t * (2.0 * Ebeam + MT)) * mv2;
+  const double A1 = 8.0 * MT * MT / ut;
+  const double Am1 = (8.0 / ut) *
+                     (MT * MT *
+                          (2.0 * t * ut + ut * ut +
+                           4.0 * Ebeam * Ebeam * (2.0 * (x - 1.0) * mv2 - t * ((x - 2.0) * x + 2.0)) +
+                           2.0 * t * (-(mV * mV) + 2.0 * m_lepton * m_lepton + t)) -
+                      2.0 * Ebeam * MT * t * ((1.0 - x) * ut + (x - 2.0) * (mv2 + t)) + t * t * (ut - mV * mV));
+  const double A0 = (8.0 / (ut * ut)) * (MT * MT * (2.0 * t * ut + (t - 4.0 * Ebeam * Ebeam * (x - 1.0) * (x - 1.0)) * mv2) +
+                                        2.0 * Ebeam * MT * t * (ut - (x - 1.0) * mv2));
+  const double Y = -t + 2.0 * q0 * Ebeam - 2.0 * q * p * (p - k * ct) * cq / V;
+  const double W = fabs(Y * Y - 4.0 * q * q * p * p * k * k * (1.0 - ct * ct) * (1.0 - cq * cq) / (V * V));
+  const bool ok = (x * Ebeam >= mV) && (discr >= 0.0) && (tplus > tminus) && (t > tminus) && (t < tplus) &&
+                  (fabs(cq) <= 1.0) && (W > 0.0);
+  if (!ok) return 0.0;
+  const double phi_int = (A0 + Y * A1 + Am1 / sqrt(W) + Y * Am2 / (W * sqrt(W))) / (8.0 * MT * MT);
+  const double ff = g_el_inel_over_t2(Z, A, t);
+  const double ans = ff * kAlpha * kAlpha * kAlpha * k * Ebeam * phi_int / (p * sqrt(k * k + p * p - 2.0 * p * k * ct));
+  const double out = ans * tconv * Jac;
+  return isfinite(out) ? out : 0.0;
+}
+
+// Kuraev-Fadin structure functions, radiative_return.py:26-44, 61-79.
+__device__ inline double fl_kf(double x, double s) {
+  const double beta = (2.0 * kAlpha / kPi) * (log(s / (kMe * kMe)) - 1.0);
+  if (x >= 1.0) x = 1.0 - 1e-10;
+  return (beta / 16.0) * ((8.0 + 3.0 * beta) * pow(1.0 - x, beta / 2.0 - 1.0) - 4.0 * (1.0 + x));
+}
+__device__ inline double fl_kf_scaled(double x, double s) {
+  const double beta = (2.0 * kAlpha / kPi) * (log(s / (kMe * kMe)) - 1.0);
+  return (beta / 16.0) * ((8.0 + 3.0 * beta) - 4.0 * (1.0 + x) * pow(1.0 - x, 1.0 - beta / 2.0));
+}
+
+// DarkAnn radiative return in u, all_processes.py:400-466.
+__device__ inline double f_dark_ann(double Ee, double mV, double u0) {
+  const double s = 2.0 * kMe * (Ee + kMe);
+  if (s < mV * mV) return 0.0;
+  const double beta = (2.0 * kAlpha / kPi) * (log(s / (kMe * kMe)) - 1.0);
+  const double umax = pow(1.0 - mV * mV / s, beta / 2.0);
+  const double betaf = sqrt(1.0 - 4.0 * kMe * kMe / (mV * mV));
+  const double prefac = (4.0 * kPi * kPi) * kAlpha * betaf * (1.5 - betaf * betaf / 2.0) / s * umax;
+  const double x1 = 1.0 - pow(u0 * umax, 2.0 / beta);
+  const double x2 = mV * mV / (x1 * s);
+  if (!((x2 < 1.0) && (x1 > 0.0) && (u0 < 1.0))) return 0.0;
+  const double y = mV * mV / s;
+  const double xx = 1.0 - pow(u0 * umax, 2.0 / beta);
+  const double lumi = fl_kf(y / xx, s) * fl_kf_scaled(xx, s) * (1.0 / xx) * (2.0 / beta);
+  const double out = 2.0 * prefac * lumi;
+  return isfinite(out) ? out : 0.0;
+}
+
+// J*f of one candidate; x holds the mapped point.  E is the true energy.
+__device__ inline double integrand(const DevTables& T, int proc, double E, const double* x) {
+  switch (proc) {
+    case BREM: return (double)f_brem(T, E, kMe, T.brem_C_e, x);
+    case MUONBREM: return (double)f_brem(T, E, kMmu, T.brem_C_mu, x);
+    case PAIRPROD: return (double)f_pairprod(T, E, x);
+    case COMP: return f_compton(E, 0.0, x[0]);
+    case ANN: return f_annihilation(E, 0.0, T.d.Eg_min, x[0]);
+    case MOLLER: return f_moller(E, T.d.Ee_min, x[0]);
+    case BHABHA: return f_bhabha(E, T.d.Ee_min, x[0]);
+    case MUONE: return f_muone(E, T.d.Ee_min, x[0]);
+    case DARKBREM: return f_dark_brem(E, kMe, T.d.mV, T.d.mT, T.d.Z, T.d.A, x);
+    case DARKMUONBREM: return f_dark_brem(E, kMmu, T.d.mV, T.d.mT, T.d.Z, T.d.A, x);
+    case DARKANN: return f_dark_ann(E, T.d.mV, x[0]);
+    case DARKCOMP: return f_compton(E, T.d.mV, x[0]);
+    default: return 0.0;
+  }
+}
+
+}  // namespace petite

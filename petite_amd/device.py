@@ -1,0 +1,160 @@
+"""Device context: one libpetite_hip context per (material, table set, cuts).
+
+Device memory for inputs and outputs is allocated with PyTorch (ROCm) and
+handed to the C ABI as raw pointers; the context holds the table copies.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .constants import all_process_code, mass_dict
+from .tables import MaterialTables, build_blobs
+
+MAXF_SEED = 0x5EEDF00D
+MAXF_SAMPLES = 32768  # ~ n_trials (100) x neval (300) of utilities/find_maxes.py:206-210
+
+
+def _stream_handle():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("petite_amd needs a HIP device (MI355X); no GPU is visible")
+
+
+class DeviceContext:
+    def __init__(self, tables: MaterialTables, min_energy, maxF_fudge=1.0, max_trials=int(1e4) * 300, MS=True,
+                 rescale_MCS=1.0, dark=None, maxf_samples=MAXF_SAMPLES):
+        _require_gpu()
+        self.lib = _lib.load()
+        self.tables = tables
+        self.dark = dark
+        desc, b64, b32 = build_blobs(tables, dark)
+        desc.min_energy = float(min_energy)
+        desc.rescale_MCS = float(rescale_MCS)
+        desc.maxF_fudge = float(maxF_fudge)
+        desc.max_trials = int(max_trials)
+        desc.MS = 1 if MS else 0
+        if dark is not None:
+            desc.mV = float(dark.mV)
+            desc.Zeff = float(dark.Zeff)
+        self.desc = desc
+        self._b64, self._b32 = b64, b32
+        ctx = C.c_void_p()
+        _lib.check(self.lib.petite_ctx_create(C.byref(desc), b64.ctypes.data, b64.size, b32.ctypes.data, b32.size,
+                                              C.byref(ctx)), "petite_ctx_create")
+        self.ctx = ctx
+        self.maxf = {}
+        self.sigma_est = {}
+        for P, code in all_process_code.items():
+            if desc.grid[code].n_E > 0:
+                self.scan_maxf(P, maxf_samples)
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.petite_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_timing(self, on=True):
+        _lib.check(self.lib.petite_set_timing(self.ctx, 1 if on else 0), "petite_set_timing")
+
+    def scan_maxf(self, process, n_samples=MAXF_SAMPLES, seed=MAXF_SEED):
+        code = all_process_code[process]
+        n_E = self.desc.grid[code].n_E
+        mx = np.zeros(n_E)
+        mean = np.zeros(n_E)
+        _lib.check(self.lib.petite_maxf_scan(self.ctx, code, int(n_samples), int(seed), mx.ctypes.data,
+                                             mean.ctypes.data, _stream_handle()), "petite_maxf_scan")
+        self.maxf[process] = mx
+        self.sigma_est[process] = mean
+        return mx, mean
+
+    def set_maxf(self, process, values):
+        code = all_process_code[process]
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        _lib.check(self.lib.petite_set_maxf(self.ctx, code, v.ctypes.data, v.size), "petite_set_maxf")
+        self.maxf[process] = v.copy()
+
+    def draw(self, proc, E, sid, seed):
+        """Batched draws; proc/E/sid are 1-D host arrays or cuda tensors."""
+        dev = torch.device("cuda")
+        proc_t = torch.as_tensor(np.asarray(proc, dtype=np.int32) if not torch.is_tensor(proc) else proc,
+                                 device=dev, dtype=torch.int32).contiguous()
+        E_t = torch.as_tensor(np.asarray(E, dtype=np.float64) if not torch.is_tensor(E) else E,
+                              device=dev, dtype=torch.float64).contiguous()
+        if torch.is_tensor(sid):
+            sid_t = sid.to(dev).contiguous()
+        else:
+            sid_t = torch.as_tensor(np.asarray(sid, dtype=np.uint64).view(np.int64), device=dev)
+        n = proc_t.numel()
+        if not (E_t.numel() == n and sid_t.numel() == n):
+            raise ValueError("draw: proc, E and sid must have the same length")
+        x = torch.empty((n, 4), device=dev, dtype=torch.float32)
+        trials = torch.empty(n, device=dev, dtype=torch.int32)
+        w = torch.empty(n, device=dev, dtype=torch.float32)
+        rc = self.lib.petite_vegas_draw(self.ctx, proc_t.data_ptr(), E_t.data_ptr(), sid_t.data_ptr(), n, int(seed),
+                                        x.data_ptr(), trials.data_ptr(), w.data_ptr(), _stream_handle())
+        if rc not in (_lib.PETITE_OK, _lib.PETITE_ERR_NO_SAMPLE):
+            _lib.check(rc, "petite_vegas_draw")
+        return x, trials, w, rc
+
+    def eval_points(self, proc, E, words):
+        """J*f and x of single candidates given their Philox words (n x 4 uint32)."""
+        dev = torch.device("cuda")
+        proc_t = torch.as_tensor(np.asarray(proc, dtype=np.int32), device=dev)
+        E_t = torch.as_tensor(np.asarray(E, dtype=np.float64), device=dev)
+        w_np = np.ascontiguousarray(words, dtype=np.uint32).view(np.int32)
+        words_t = torch.as_tensor(w_np, device=dev)
+        n = proc_t.numel()
+        x = torch.empty((n, 4), device=dev, dtype=torch.float64)
+        w = torch.empty(n, device=dev, dtype=torch.float64)
+        _lib.check(self.lib.petite_eval_points(self.ctx, proc_t.data_ptr(), E_t.data_ptr(), words_t.data_ptr(), n,
+                                               x.data_ptr(), w.data_ptr(), _stream_handle()), "petite_eval_points")
+        return x.cpu().numpy(), w.cpu().numpy()
+
+    def run_showers(self, primaries, seed, capacity=None):
+        """primaries: structured numpy array of PRIMARY_DTYPE.  Returns (store dict of tensors, n, stats, energy)."""
+        from .batch import PRIMARY_DTYPE
+        prim = np.ascontiguousarray(primaries, dtype=PRIMARY_DTYPE)
+        n = len(prim)
+        dev = torch.device("cuda")
+        prim_t = torch.from_numpy(prim.view(np.uint8).copy()).to(dev)
+        n_showers = int(prim["shower"].max()) + 1 if n else 0
+        if capacity is None:
+            e = np.maximum(prim["E"], 0.0)
+            capacity = int(np.sum(2.5 * e / max(self.desc.min_energy, 1e-3)) + 64 * n + 1024)
+        capacity = int(min(capacity, 2 ** 31 - 1))
+        while True:
+            store = {
+                "p0": torch.empty((capacity, 4), device=dev, dtype=torch.float32),
+                "r0": torch.empty((capacity, 4), device=dev, dtype=torch.float32),
+                "pf": torch.empty((capacity, 4), device=dev, dtype=torch.float32),
+                "rf": torch.empty((capacity, 4), device=dev, dtype=torch.float32),
+                "pid": torch.empty(capacity, device=dev, dtype=torch.int32),
+                "parent": torch.empty(capacity, device=dev, dtype=torch.int32),
+                "meta": torch.empty(capacity, device=dev, dtype=torch.int32),
+                "shower": torch.empty(capacity, device=dev, dtype=torch.int32),
+                "sid": torch.empty(capacity, device=dev, dtype=torch.int64),
+            }
+            energy = torch.zeros((max(n_showers, 1), 4), device=dev, dtype=torch.float64)
+            S = _lib.Store(*(store[k].data_ptr() for k in ("p0", "r0", "pf", "rf", "pid", "parent", "meta", "shower",
+                                                            "sid")), capacity)
+            st = _lib.Stats()
+            rc = self.lib.petite_run_showers(self.ctx, prim_t.data_ptr(), n, int(seed), C.byref(S), energy.data_ptr(),
+                                             C.byref(st), _stream_handle())
+            if rc == _lib.PETITE_ERR_CAPACITY and capacity < 2 ** 31 - 1:
+                capacity = int(min(max(2 * capacity, int(st.n_particles * 1.2)), 2 ** 31 - 1))
+                del store
+                continue
+            if rc not in (_lib.PETITE_OK, _lib.PETITE_ERR_NO_SAMPLE):
+                _lib.check(rc, "petite_run_showers")
+            return store, int(st.n_particles), st.as_dict(), energy, rc

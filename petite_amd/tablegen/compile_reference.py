@@ -2,7 +2,7 @@
 
 Run once, in a checkout where the reference tree is readable::
 
-    python -m petite_amd.tables.compile_reference /path/to/PETITE
+    python -m petite_amd.tablegen.compile_reference /path/to/PETITE
 
 The reference's ``data/`` (energies up to 100 GeV) and ``data_400GeV/``
 (up to 400 GeV) directories hold, per table set (layout documented by
@@ -15,7 +15,7 @@ utilities/find_maxes.py:112-119,203-206,302-305):
 * ``Dark<P>/mV_<k>MeV/Dark<P>_AdaptiveMaps.npy``
 * ``dark_weights.pkl`` / ``dark_drate.pkl`` caches (partial coverage)
 
-All are read with :mod:`petite_amd.tables.static_pickle`, which executes
+All are read with :mod:`petite_amd.tablegen.static_pickle`, which executes
 nothing from the files.  VEGAS grid nodes are stored as float32 (the device
 format; a map only changes sampling efficiency, its end points are kept to
 float32 rounding); cross sections and weights stay float64.
