@@ -38,6 +38,8 @@ typedef struct {
   int64_t E_off;    /* offset into the float64 blob: n_E grid energies */
   int64_t node_off; /* offset into the float32 blob: n_E * stride nodes */
   int64_t stride;   /* nodes per energy = sum_d (ninc[d] + 1) */
+  double lnE0;      /* geometric energy grid: ln E[0] and (n_E-1)/ln(E[n-1]/E[0]); */
+  double inv_dlog;  /* inv_dlog = 0 for a grid that is not geometric (binary search) */
 } PetiteGridDesc;
 
 /* Linear n*sigma table [1/cm], zero outside [E[0], E[n-1]]. */
@@ -46,6 +48,8 @@ typedef struct {
   int32_t pad;
   int64_t E_off; /* float64 blob offsets */
   int64_t v_off;
+  double lnE0;     /* as in PetiteGridDesc */
+  double inv_dlog;
 } PetiteSigmaDesc;
 
 /* Material, cuts and tables (src/PETITE/shower.py:100-142, 189-246, 273-295). */

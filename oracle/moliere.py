@@ -4,8 +4,9 @@ Restates src/PETITE/moliere.py:196-219 (chi_c^2, chi_a^2), 265-284
 (generate_moliere_angle_simplified_alt), 287-348 (get_rotation_matrix) and
 350-400 (get_scattered_momentum_fast).  Random numbers: the reference's
 random.choice([-1,1]) * sqrt(gauss^2 + gauss^2) is a signed Rayleigh(1)
-variate; it is drawn here as sign(u1 - 1/2) * sqrt(-2 ln u0), and the
-azimuth as 2 pi u2, from one Philox block (PURPOSE_MCS / PURPOSE_MCS_CLOSE).
+variate; it is drawn here as s * sqrt(-2 ln u_r), with the azimuth 2 pi u_a
+and the sign s = +1 when bit 0 of the azimuth word is set (words of the
+substep's or the closing increment's Philox block, see oracle.shower).
 
 Pinned by: Rayleigh law of the space angle (KS) and |p| preservation
 (tests/test_oracle_moliere.py).
@@ -78,8 +79,8 @@ def rotation_to_z(v):
     return Rb @ Ra
 
 
-def scatter(p4, t, A, Z, rescale, m_lepton, u):
-    """get_scattered_momentum_fast (moliere.py:350-400) with uniforms u[0..2]."""
+def scatter(p4, t, A, Z, rescale, m_lepton, u_r, u_a, sign):
+    """get_scattered_momentum_fast (moliere.py:350-400) with radius/azimuth uniforms and a sign."""
     p3 = np.asarray(p4[1:], dtype=np.float64)
     pn = float(np.linalg.norm(p3))
     if not pn > 0:
@@ -87,9 +88,8 @@ def scatter(p4, t, A, Z, rescale, m_lepton, u):
     beta = pn / p4[0]
     Rz = rotation_to_z(p3).T
     th0 = theta0(t, beta, A, Z, m_lepton)
-    sign = -1.0 if u[1] < 0.5 else 1.0
-    theta = sign * math.sqrt(-2.0 * math.log(u[0])) * th0 * rescale
-    phi = 2.0 * math.pi * u[2]
+    theta = sign * math.sqrt(-2.0 * math.log(u_r)) * th0 * rescale
+    phi = 2.0 * math.pi * u_a
     cth, sth, cph, sph = math.cos(theta), math.sin(theta), math.cos(phi), math.sin(phi)
     local = pn * np.array([sph * sth, -cph * sth, cth])
     out = np.empty(4)

@@ -21,9 +21,8 @@ W0 = np.uint32(0x9E3779B9)
 W1 = np.uint32(0xBB67AE85)
 MASK32 = np.uint64(0xFFFFFFFF)
 
-PURPOSE_STEP = 0       # transport substep k: u0 hard test, u1 -> U(6,20), u2 closing fraction
-PURPOSE_MCS = 1        # MCS after substep k: u0 radius, u1 sign, u2 azimuth
-PURPOSE_MCS_CLOSE = 2  # MCS of the closing increment (index 0)
+PURPOSE_STEP = 0       # substep k: x hard test, y -> U(6,20), z MCS radius, w MCS azimuth (sign: bit 0 of w)
+PURPOSE_CLOSE = 2      # closing increment: x fraction, y MCS radius, z MCS azimuth (sign: bit 0 of z)
 PURPOSE_INTERACT = 3   # index 0: u0 process choice, u1 kinematic azimuth, u2/u3 extra
 PURPOSE_VEGAS_A = 4    # candidate j: y0..y3 (ndim<=3: u3 is the accept uniform)
 PURPOSE_VEGAS_B = 5    # candidate j (ndim 4): u0 accept uniform
@@ -56,6 +55,20 @@ def to_uniform(x):
     """uint32 -> float64 uniform in (0,1), exact in float32 as well."""
     x = np.asarray(x, dtype=np.uint32)
     return ((x >> np.uint32(9)).astype(np.float64) + 0.5) * (2.0 ** -23)
+
+
+def words(seed, sid, index, purpose):
+    """The four raw 32-bit words of one Philox block."""
+    sid = np.asarray(sid, dtype=np.uint64)
+    index = np.asarray(index, dtype=np.uint64)
+    shape = np.broadcast(sid, index).shape
+    ctr = np.empty(shape + (4,), dtype=np.uint32)
+    ctr[..., 0] = (np.broadcast_to(sid, shape) & MASK32).astype(np.uint32)
+    ctr[..., 1] = (np.broadcast_to(sid, shape) >> np.uint64(32)).astype(np.uint32)
+    ctr[..., 2] = (np.broadcast_to(index, shape) & MASK32).astype(np.uint32)
+    ctr[..., 3] = np.uint32(purpose)
+    key = np.array([int(seed) & 0xFFFFFFFF, (int(seed) >> 32) & 0xFFFFFFFF], dtype=np.uint32)
+    return philox4x32_10(ctr, key)
 
 
 def block(seed, sid, index, purpose):

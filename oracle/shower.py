@@ -134,7 +134,8 @@ class OracleShower:
         delta_z = 0.0
         while not hard and pf[0] >= pme:
             mfp = T.mfp(p.pid, pf[0])
-            u = philox.block(seed, p.sid, k, philox.PURPOSE_STEP)
+            w = philox.words(seed, p.sid, k, philox.PURPOSE_STEP)
+            u = philox.to_uniform(w)
             delta_z = mfp / (6.0 + 14.0 * u[1])
             if u[0] > math.exp(-delta_z / mfp):
                 hard = True
@@ -146,10 +147,12 @@ class OracleShower:
                 if pn > 0.0:
                     rf = rf + pf[1:] / pn * delta_z
                 if self.MS:
-                    um = philox.block(seed, p.sid, k, philox.PURPOSE_MCS)
-                    pf = scatter(pf, T.rho * (delta_z / cmtom), T.A, T.Z, self.rescale, mass_dict[p.pid], um)
+                    pf = scatter(pf, T.rho * (delta_z / cmtom), T.A, T.Z, self.rescale, mass_dict[p.pid], u[2], u[3],
+                                 1.0 if int(w[3]) & 1 else -1.0)
             k += 1
-        distC = philox.block(seed, p.sid, k - 1, philox.PURPOSE_STEP)[2]
+        wc = philox.words(seed, p.sid, 0, philox.PURPOSE_CLOSE)
+        uc = philox.to_uniform(wc)
+        distC = uc[0]
         if pf[0] < pme:
             last = distC * delta_z
         else:
@@ -162,8 +165,8 @@ class OracleShower:
         if pn > 0.0:
             rf = rf + pf[1:] / pn * last
         if self.MS:
-            um = philox.block(seed, p.sid, 0, philox.PURPOSE_MCS_CLOSE)
-            pf = scatter(pf, T.rho * (last / cmtom), T.A, T.Z, self.rescale, m_electron, um)
+            pf = scatter(pf, T.rho * (last / cmtom), T.A, T.Z, self.rescale, m_electron, uc[1], uc[2],
+                         1.0 if int(wc[2]) & 1 else -1.0)
         p.pf, p.rf = pf, rf
         p.steps = k + 1
         st.steps += k + 1

@@ -23,11 +23,12 @@ EXPORTED = ["petite_ctx_create", "petite_ctx_destroy", "petite_last_error", "pet
 
 class GridDesc(C.Structure):
     _fields_ = [("n_E", C.c_int32), ("ndim", C.c_int32), ("ninc", C.c_int32 * 4), ("E_off", C.c_int64),
-                ("node_off", C.c_int64), ("stride", C.c_int64)]
+                ("node_off", C.c_int64), ("stride", C.c_int64), ("lnE0", C.c_double), ("inv_dlog", C.c_double)]
 
 
 class SigmaDesc(C.Structure):
-    _fields_ = [("n", C.c_int32), ("pad", C.c_int32), ("E_off", C.c_int64), ("v_off", C.c_int64)]
+    _fields_ = [("n", C.c_int32), ("pad", C.c_int32), ("E_off", C.c_int64), ("v_off", C.c_int64),
+                ("lnE0", C.c_double), ("inv_dlog", C.c_double)]
 
 
 class TableDesc(C.Structure):

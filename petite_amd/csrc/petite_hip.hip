@@ -38,24 +38,34 @@ int fail(int code, const std::string& msg) {
 
 enum StatIdx { ST_STEPS = 0, ST_DRAWS, ST_TRIALS, ST_VIOL, ST_NOSAMPLE, ST_DROPPED, ST_OVERFLOW, ST_N = 8 };
 
-struct DrawIO {
-  // explicit requests (petite_vegas_draw)
-  const int32_t* proc;
-  const double* E;
-  const uint64_t* sid;
-  // shower pipeline requests: per-process regions of req_cap entries
-  const int32_t* req_slot;
+// Layout of the context's counter block (unsigned long long).
+enum CounterIdx {
+  C_REFILL = 0,               // [PETITE_NPROC] per-process draw refill counters
+  C_TRANSPORT = 16,           // transport refill counter
+  C_STORE = 17,               // particles written to the store
+  C_STATS = 32,               // [ST_N] run statistics
+  C_PTRIALS = 64,             // [PETITE_NPROC] candidates per process
+  C_PDRAWS = 80,              // [PETITE_NPROC] draws per process
+  C_N = 96
+};
+
+// One launch of the accept/reject kernel.  Requests sit in per-process
+// regions (region p holds count[p] entries from p * req_cap); each block
+// serves one process, so the process, its grid descriptor and its
+// integrand are wave-uniform.
+struct DrawLaunch {
+  const int32_t* req_slot;   // store slot (pipeline) or request index (explicit API)
   const double* req_E;
-  const uint64_t* store_sid;
+  const uint64_t* sid_src;   // stream ids, indexed by req_slot
   int64_t req_cap;
-  int64_t seg[PETITE_NPROC + 1];
-  // results (explicit: index r; pipeline: p * req_cap + i)
+  int32_t count[PETITE_NPROC];
+  int32_t block_start[PETITE_NPROC + 1];
+  int32_t lgG[PETITE_NPROC];  // log2 of the lanes that share one request
+  int32_t explicit_out;       // write results at index req_slot instead of the region index
   float* x_out;
   int32_t* trials_out;
   float* w_out;
-  int64_t total;
-  unsigned long long* counter;
-  unsigned long long* stats;
+  unsigned long long* counters;
 };
 
 __device__ inline unsigned long long shfl_u64(unsigned long long v, int src) {
@@ -63,117 +73,221 @@ __device__ inline unsigned long long shfl_u64(unsigned long long v, int src) {
   const unsigned hi = __shfl((unsigned)(v >> 32), src);
   return ((unsigned long long)hi << 32) | lo;
 }
+__device__ inline double shfl_f64(double v, int src) {
+  return __longlong_as_double((long long)shfl_u64((unsigned long long)__double_as_longlong(v), src));
+}
 
 __device__ inline unsigned long long wave_sum(unsigned long long v) {
   for (int off = 32; off > 0; off >>= 1) v += shfl_u64(v, (threadIdx.x & 63) ^ off);
   return v;
 }
 
-__device__ inline void wave_stat(unsigned long long* stats, int idx, unsigned long long v) {
+__device__ inline void wave_stat(unsigned long long* ctr, int idx, unsigned long long v) {
   v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(stats + idx, v);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr + idx, v);
+}
+
+// Hands out request indices to the lanes in ``need`` from wave-private
+// chunks of CH consecutive indices (one atomic per chunk).
+template <int CH>
+struct WaveQueue {
+  int64_t cur = 0, end = 0;
+  __device__ inline int64_t take(unsigned long long need_mask, bool need, unsigned long long* counter) {
+    const int lane = threadIdx.x & 63;
+    const int n = __popcll(need_mask);
+    const int64_t avail = end - cur;
+    unsigned long long nb = 0;
+    if (avail < n) {
+      if (lane == 0) nb = atomicAdd(counter, (unsigned long long)CH);
+      nb = shfl_u64(nb, 0);
+    }
+    int64_t r = -1;
+    if (need) {
+      const int64_t rank = __popcll(need_mask & ((1ull << lane) - 1ull));
+      r = rank < avail ? cur + rank : (int64_t)nb + (rank - avail);
+    }
+    if (avail < n) {
+      cur = (int64_t)nb + (n - avail);
+      end = (int64_t)nb + CH;
+    } else {
+      cur += n;
+    }
+    return r;
+  }
+};
+
+template <int P>
+__device__ inline double integrand_p(const DevTables& T, double E, const double* x) {
+  if (P == BREM) return (double)f_brem(T, E, kMe, T.brem_C_e, x);
+  if (P == MUONBREM) return (double)f_brem(T, E, kMmu, T.brem_C_mu, x);
+  if (P == PAIRPROD) return (double)f_pairprod(T, E, x);
+  if (P == COMP) return f_compton(E, 0.0, x[0]);
+  if (P == ANN) return f_annihilation(E, 0.0, T.d->Eg_min, x[0]);
+  if (P == MOLLER) return f_moller(E, T.d->Ee_min, x[0]);
+  if (P == BHABHA) return f_bhabha(E, T.d->Ee_min, x[0]);
+  if (P == MUONE) return f_muone(E, T.d->Ee_min, x[0]);
+  if (P == DARKBREM) return f_dark_brem(E, kMe, T.d->mV, T.d->mT, T.d->Z, T.d->A, x);
+  if (P == DARKMUONBREM) return f_dark_brem(E, kMmu, T.d->mV, T.d->mT, T.d->Z, T.d->A, x);
+  if (P == DARKANN) return f_dark_ann(E, T.d->mV, x[0]);
+  if (P == DARKCOMP) return f_compton(E, T.d->mV, x[0]);
+  return 0.0;
+}
+
+template <int P>
+constexpr int proc_ndim() {
+  return (P == BREM || P == MUONBREM || P == PAIRPROD) ? 4 : ((P == DARKBREM || P == DARKMUONBREM) ? 3 : 1);
+}
+
+template <int P, int ND>
+__device__ inline double candidate_w(const DevTables& T, double E, const float* base, const int32_t* ninc,
+                                     uint64_t seed, uint64_t sid, uint32_t j, double* x, double& ua) {
+  const U4 a = draw_block(seed, sid, j, PURPOSE_VEGAS_A);
+  const uint32_t rr[4] = {a.x, a.y, a.z, a.w};
+  if (ND == 4) {
+    ua = u01d(draw_block(seed, sid, j, PURPOSE_VEGAS_B).x);
+  } else {
+    ua = u01d(a.w);
+  }
+  const double J = map_point_n<ND>(base, ninc, rr, x);
+  return J * integrand_p<P>(T, E, x);
 }
 
 // ------------------------------------------------------------ draw kernel --
-// Lane-level refill: a lane that accepts (or exhausts max_trials) takes the
-// next request at once, so every lane evaluates candidates until the request
-// list is drained.  Candidate j of a request uses Philox blocks (sid, j,
-// VEGAS_A/B), which makes the accepted point equal to the first accepted
-// candidate of the sequential loop of draw_sample.
-template <bool PIPE>
-__global__ __launch_bounds__(256) void k_draw(DevTables T, DrawIO io, uint64_t seed) {
+// Groups of G = 2^lgG lanes work on one request: lane g of a group tests
+// candidate j + g, and the group takes the lowest accepted index, which is
+// the candidate the sequential loop of draw_sample accepts (candidate j
+// always uses Philox counter j).  G = 1 keeps every lane on its own request
+// and refills it as soon as it accepts; larger G cuts the latency of sweeps
+// with fewer requests than lanes.
+template <int p>
+__device__ void draw_loop(const DevTables& T, const DrawLaunch& io, uint64_t seed) {
+  constexpr int ND = proc_ndim<p>();
   const int lane = threadIdx.x & 63;
+  const int lgG = io.lgG[p];
+  const int G = 1 << lgG;
+  const int gl = lane & (G - 1);
+  const int g0 = lane - gl;
+  const unsigned long long gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << g0);
+  const PetiteGridDesc& gd = T.d->grid[p];
+  const int32_t ninc[4] = {gd.ninc[0], gd.ninc[1], gd.ninc[2], gd.ninc[3]};
+  const int64_t count = io.count[p];
+  const uint32_t max_trials = (uint32_t)(T.d->max_trials < (int64_t)0x7FFFFFF0 ? (T.d->max_trials > 0 ? T.d->max_trials : 1)
+                                                                               : (int64_t)0x7FFFFFF0);
+  WaveQueue<64> q;
   bool have = false, done = false;
-  int proc = 0, ie = 0, ndim = 1;
   double E = 0.0, mf = 0.0;
   uint64_t sid = 0;
+  const float* base = T.f32;
   uint32_t j = 0;
   int64_t oidx = 0;
   unsigned long long n_trials = 0, n_viol = 0, n_fail = 0, n_draw = 0;
-  const uint32_t max_trials = (uint32_t)(T.d.max_trials < (int64_t)0xFFFFFFF0 ? (T.d.max_trials > 0 ? T.d.max_trials : 1)
-                                                                                : (int64_t)0xFFFFFFF0);
   while (true) {
-    const bool need = !have && !done;
-    const unsigned long long mask = __ballot(need);
-    if (mask) {
-      const int n = __popcll(mask);
-      const int leader = __ffsll((long long)mask) - 1;
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd(io.counter, (unsigned long long)n);
-      base = shfl_u64(base, leader);
+    const bool need = (gl == 0) && !have && !done;
+    const unsigned long long nm = __ballot(need);
+    if (nm) {
+      const int64_t r = q.take(nm, need, io.counters + C_REFILL + p);
       if (need) {
-        const unsigned long long rank = __popcll(mask & ((1ull << lane) - 1ull));
-        const int64_t r = (int64_t)(base + rank);
-        if (r < io.total) {
-          if (PIPE) {
-            int p = 0;
-            while (r >= io.seg[p + 1]) ++p;
-            const int64_t i = r - io.seg[p];
-            oidx = (int64_t)p * io.req_cap + i;
-            proc = p;
-            E = io.req_E[oidx];
-            sid = io.store_sid[io.req_slot[oidx]];
-          } else {
-            oidx = r;
-            proc = io.proc[r];
-            E = io.E[r];
-            sid = io.sid[r];
-          }
-          if (proc < 0 || proc >= PETITE_NPROC || T.d.grid[proc].n_E <= 0) {
-            // no grid for this process: report a failed draw without sampling
-            io.trials_out[oidx] = 0;
-            io.w_out[oidx] = 0.0f;
-            ++n_fail;
-          } else {
-            ndim = T.d.grid[proc].ndim;
-            ie = lu_key(T, proc, E);
-            mf = T.maxf[proc * T.maxE + ie] * T.d.maxF_fudge;
-            j = 0;
-            have = true;
-          }
+        if (r < count) {
+          const int64_t ridx = (int64_t)p * io.req_cap + r;
+          const int32_t src = io.req_slot[ridx];
+          oidx = io.explicit_out ? (int64_t)src : ridx;
+          E = io.req_E[ridx];
+          sid = io.sid_src[src];
+          const int ie = lu_key(T, p, E);
+          base = T.f32 + gd.node_off + (int64_t)ie * gd.stride;
+          mf = T.maxf[p * T.maxE + ie] * T.d->maxF_fudge;
+          j = 0;
+          have = true;
         } else {
           done = true;
         }
       }
+      if (G > 1) {
+        // the group leader's request becomes the group's
+        have = __shfl((int)have, g0) != 0;
+        done = __shfl((int)done, g0) != 0;
+        E = shfl_f64(E, g0);
+        mf = shfl_f64(mf, g0);
+        sid = shfl_u64(sid, g0);
+        oidx = (int64_t)shfl_u64((unsigned long long)oidx, g0);
+        base = (const float*)shfl_u64((unsigned long long)base, g0);
+        j = __shfl(j, g0);
+      }
     }
     if (__ballot(have) == 0ull) break;
+    double x[4] = {0.0, 0.0, 0.0, 0.0};
+    double w = 0.0, ua = 1.0;
+    const uint32_t jc = j + (uint32_t)gl;
+    if (have) w = candidate_w<p, ND>(T, E, base, ninc, seed, sid, jc, x, ua);
+    const bool acc = have && (jc < max_trials) && (mf * ua < w);
+    const unsigned long long am = __ballot(acc) & gmask;
     if (have) {
-      const U4 a = draw_block(seed, sid, j, PURPOSE_VEGAS_A);
-      const uint32_t rr[4] = {a.x, a.y, a.z, a.w};
-      double ua;
-      if (ndim == 4) {
-        const U4 b = draw_block(seed, sid, j, PURPOSE_VEGAS_B);
-        ua = u01d(b.x);
-      } else {
-        ua = u01d(a.w);
-      }
-      double x[4] = {0.0, 0.0, 0.0, 0.0};
-      const double J = map_point(T, proc, ie, rr, ndim, x);
-      const double w = J * integrand(T, proc, E, x);
-      ++j;
-      const bool acc = mf * ua < w;
-      if (acc || j >= max_trials) {
-        float* xo = io.x_out + 4 * oidx;
-        xo[0] = (float)x[0];
-        xo[1] = (float)x[1];
-        xo[2] = (float)x[2];
-        xo[3] = (float)x[3];
-        io.trials_out[oidx] = acc ? (int32_t)j : -(int32_t)j;
-        io.w_out[oidx] = acc ? (float)w : 0.0f;
-        n_trials += j;
-        n_draw += 1;
-        if (acc && w > mf) ++n_viol;
-        if (!acc) ++n_fail;
+      if (am) {
+        const int win = __ffsll((long long)am) - 1;
+        if (lane == win) {
+          float* xo = io.x_out + 4 * oidx;
+          xo[0] = (float)x[0];
+          xo[1] = (float)x[1];
+          xo[2] = (float)x[2];
+          xo[3] = (float)x[3];
+          io.trials_out[oidx] = (int32_t)(jc + 1);
+          io.w_out[oidx] = (float)w;
+          n_trials += jc + 1;
+          n_draw += 1;
+          if (w > mf) ++n_viol;
+        }
         have = false;
+      } else {
+        j += (uint32_t)G;
+        if (j >= max_trials) {
+          if (gl == 0) {
+            io.trials_out[oidx] = -(int32_t)max_trials;
+            io.w_out[oidx] = 0.0f;
+            n_trials += max_trials;
+            n_draw += 1;
+            ++n_fail;
+          }
+          have = false;
+        }
       }
     }
   }
-  if (io.stats) {
-    wave_stat(io.stats, ST_TRIALS, n_trials);
-    wave_stat(io.stats, ST_DRAWS, n_draw);
-    wave_stat(io.stats, ST_VIOL, n_viol);
-    wave_stat(io.stats, ST_NOSAMPLE, n_fail);
+  wave_stat(io.counters, C_STATS + ST_TRIALS, n_trials);
+  wave_stat(io.counters, C_STATS + ST_DRAWS, n_draw);
+  wave_stat(io.counters, C_STATS + ST_VIOL, n_viol);
+  wave_stat(io.counters, C_STATS + ST_NOSAMPLE, n_fail);
+  wave_stat(io.counters, C_PTRIALS + p, n_trials);
+  wave_stat(io.counters, C_PDRAWS + p, n_draw);
+}
+
+// One kernel per process: the integrand, the map dimension and the
+// register allocation are specialised at compile time.  Launch p uses the
+// blocks [block_start[p], block_start[p+1]) of the plan, renumbered from 0.
+template <int P>
+__global__ __launch_bounds__(256) void k_draw(DevTables T, DrawLaunch io, uint64_t seed) {
+  if (io.count[P] <= 0) return;
+  draw_loop<P>(T, io, seed);
+}
+
+typedef void (*DrawKernel)(DevTables, DrawLaunch, uint64_t);
+static const DrawKernel kDrawKernels[PETITE_NPROC] = {
+    k_draw<0>, k_draw<1>, k_draw<2>, k_draw<3>, k_draw<4>, k_draw<5>,
+    k_draw<6>, k_draw<7>, k_draw<8>, k_draw<9>, k_draw<10>, k_draw<11>};
+
+// Explicit requests into per-process regions (petite_vegas_draw).
+__global__ void k_bucket(DevTables T, const int32_t* proc, const double* E, int64_t n, int32_t* req_slot, double* req_E,
+                         int64_t req_cap, unsigned int* req_count, int32_t* trials_out, float* w_out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int p = proc[i];
+  if (p < 0 || p >= PETITE_NPROC || T.d->grid[p].n_E <= 0) {
+    trials_out[i] = 0;
+    w_out[i] = 0.0f;
+    return;
   }
+  const unsigned k = atomicAdd(req_count + p, 1u);
+  req_slot[(int64_t)p * req_cap + k] = (int32_t)i;
+  req_E[(int64_t)p * req_cap + k] = E[i];
 }
 
 // ------------------------------------------------------------- max_F scan --
@@ -182,7 +296,7 @@ __global__ __launch_bounds__(256) void k_draw(DevTables T, DrawIO io, uint64_t s
 __global__ __launch_bounds__(256) void k_maxf_scan(DevTables T, int proc, int n_samples, uint64_t seed,
                                                    double* maxf_out, double* mean_out) {
   const int ie = blockIdx.x;
-  const PetiteGridDesc& g = T.d.grid[proc];
+  const PetiteGridDesc& g = T.d->grid[proc];
   const double E = T.f64[g.E_off + ie];
   const uint64_t sid = ((uint64_t)proc << 32) | (uint64_t)ie;
   double mx = 0.0, sum = 0.0;
@@ -222,10 +336,10 @@ __global__ __launch_bounds__(256) void k_eval(DevTables T, const int32_t* proc, 
   const int p = proc[i];
   double x[4] = {0.0, 0.0, 0.0, 0.0};
   double w = 0.0;
-  if (p >= 0 && p < PETITE_NPROC && T.d.grid[p].n_E > 0) {
+  if (p >= 0 && p < PETITE_NPROC && T.d->grid[p].n_E > 0) {
     const uint32_t rr[4] = {words[4 * i], words[4 * i + 1], words[4 * i + 2], words[4 * i + 3]};
     const int ie = lu_key(T, p, E[i]);
-    const double J = map_point(T, p, ie, rr, T.d.grid[p].ndim, x);
+    const double J = map_point(T, p, ie, rr, T.d->grid[p].ndim, x);
     w = J * integrand(T, p, E[i], x);
   }
   for (int d = 0; d < 4; ++d) x_out[4 * i + d] = x[d];
@@ -263,11 +377,11 @@ __global__ void k_init_primaries(DevStore S, const PetitePrimary* prim, int64_t 
 
 __device__ inline double min_calc(const DevTables& T, int pid) {
   switch (pid) {
-    case 11: return T.d.min_calc[0];
-    case -11: return T.d.min_calc[1];
-    case 22: return T.d.min_calc[2];
-    case 13: return T.d.min_calc[3];
-    case -13: return T.d.min_calc[4];
+    case 11: return T.d->min_calc[0];
+    case -11: return T.d->min_calc[1];
+    case 22: return T.d->min_calc[2];
+    case 13: return T.d->min_calc[3];
+    case -13: return T.d->min_calc[4];
     default: return 0.0;
   }
 }
@@ -279,19 +393,47 @@ struct Requests {
   unsigned int* count;  // [PETITE_NPROC]
 };
 
-// Lynch-Dahl scattering of direction (nx,ny,nz) for a step of t g/cm^2.
+// n*sigma tables in LDS.  Header per process p (4 doubles): ln E0,
+// (n-1)/ln(E_last/E0), offset of the energies, n; values follow the energies.
+struct LdsSigma {
+  const double* base;
+  __device__ inline double nsigma(int p, double E) const {
+    const double* h = base + 4 * p;
+    const int n = (int)h[3];
+    if (n <= 0) return 0.0;
+    const double* xe = base + (int)h[2];
+    const double* ye = xe + n;
+    if (!(E >= xe[0]) || E > xe[n - 1]) return 0.0;
+    if (n == 1) return ye[0];
+    const int lo = grid_interval(xe, n, h[0], h[1], E);
+    const double t = (E - xe[lo]) / (xe[lo + 1] - xe[lo]);
+    return ye[lo] + t * (ye[lo + 1] - ye[lo]);
+  }
+  __device__ inline double mfp(int pid, double E) const {
+    int ch[3];
+    const int n = channels(pid, ch);
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += nsigma(ch[i], E);
+    return s > 0.0 ? kCmToM / s : 1.0e12;
+  }
+};
+
+// Lynch-Dahl scattering of the unit direction (nx,ny,nz) over ``dz`` metres:
+// radius and azimuth from two Philox words, the sign from bit 0 of the
+// azimuth word (moliere.py:265-284, 350-400).
 __device__ inline void mcs_rotate(const DevTables& T, float& nx, float& ny, float& nz, double E, double p, double m_part,
-                                  double m_lep, double dz, const U4& u) {
+                                  double m_lep, double dz, uint32_t w_r, uint32_t w_a) {
   const float beta = (float)(p / E);
-  const float pMeV = (float)(m_lep / 1e-3 * p / m_part);
-  const float t = (float)(T.d.rho * (dz / kCmToM));
-  const float th0 = lynch_dahl_theta0(t, pMeV, beta, (float)T.d.A, (float)T.d.Z);
-  const float r = sqrtf(-2.0f * logf(u01f(u.x)));
-  const float sign = u01f(u.y) < 0.5f ? -1.0f : 1.0f;
-  const float theta = sign * r * th0 * (float)T.d.rescale_MCS;
-  float sth, cth, sph, cph;
+  const float pMeV = (float)(m_lep * 1.0e3 * (p / m_part));
+  const float t = (float)(T.d->rho * (dz / kCmToM));
+  const float th0 = lynch_dahl_theta0(T.mcs_cA, T.mcs_cB, T.mcs_cC, t, pMeV, beta);
+  const float r = sqrtf(-2.0f * logf(u01f(w_r)));
+  const float theta = ((w_a & 1u) ? r : -r) * th0 * (float)T.d->rescale_MCS;
+  float sth, cth;
   sincosf(theta, &sth, &cth);
-  sincosf(2.0f * (float)kPi * u01f(u.z), &sph, &cph);
+  const float ua = u01f(w_a);
+  const float sph = __builtin_amdgcn_sinf(ua);  // sin(2 pi ua): the hardware sine takes revolutions
+  const float cph = __builtin_amdgcn_cosf(ua);
   const Frame f = frame_of(nx, ny, nz);
   float ox, oy, oz;
   frame_apply(f, sph * sth, -cph * sth, cth, ox, oy, oz);
@@ -301,155 +443,205 @@ __device__ inline void mcs_rotate(const DevTables& T, float& nx, float& ny, floa
   nz = oz * inv;
 }
 
+// Moves a charged particle by ``dist`` after losing dEdx*dist (lose_energy,
+// particle.py:143-153) and scatters it (shower.py:569-581, 589-598).
+__device__ inline void charged_advance(const DevTables& T, double dist, double m, double m_lep, uint32_t w_r,
+                                       uint32_t w_a, double& E, double& p, float& nx, float& ny, float& nz, float& rx,
+                                       float& ry, float& rz, double& dep) {
+  const double Eb = E;
+  E = E - T.d->dEdx * dist;
+  if (E <= m) E = m;
+  p = sqrt(E * E - m * m);
+  dep += Eb - E;
+  if (p > 0.0) {
+    rx += (float)(nx * dist);
+    ry += (float)(ny * dist);
+    rz += (float)(nz * dist);
+    if (T.d->MS) mcs_rotate(T, nx, ny, nz, E, p, m, m_lep, dist, w_r, w_a);
+  } else {
+    nx = ny = nz = 0.0f;
+  }
+}
+
 // ---------------------------------------------------------- transport -----
-// propagate_particle (shower.py:509-601) + the process choice (665-698) for
-// the store slots [first, first + n).
+// propagate_particle (shower.py:509-601) and the process choice (665-698)
+// for the store slots [first, first + n).  One substep per loop iteration;
+// lanes whose particle ended take new particles once a quarter of the wave
+// is idle (or all of it), so uneven substep counts do not idle the wave and
+// the record loads of new particles are paid once per batch.
 __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int64_t first, int64_t n, uint64_t seed,
-                                                   Requests R, double* shower_energy, unsigned long long* stats) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = i < n;
+                                                   Requests R, double* shower_energy, unsigned long long* counters,
+                                                   const double* sig_lds_src) {
+  HIP_DYNAMIC_SHARED(double, s_sig)
+  for (int i = threadIdx.x; i < T.lds_doubles; i += blockDim.x) s_sig[i] = sig_lds_src[i];
+  __syncthreads();
+  const LdsSigma L{s_sig};
   const int lane = threadIdx.x & 63;
-  int want = -1;
-  double E_req = 0.0;
-  int64_t slot = first + i;
+  WaveQueue<64> q;
+  // lane states: stepping (act), finished and parked (wait), empty (neither), no work left (done)
+  bool act = false, wait = false, done = false;
+  int64_t slot = 0;
+  int pid = 0, shower = 0;
+  uint64_t sid = 0;
+  double m = 0.0, pme = 0.0, E = 0.0, p = 0.0, dep = 0.0;
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f, rx = 0.0f, ry = 0.0f, rz = 0.0f;
+  uint32_t k = 0;
   unsigned long long steps = 0;
-  if (valid) {
-    const float4 p0 = S.p0[slot];
-    const float4 r0 = S.r0[slot];
-    const int pid = S.pid[slot];
-    const uint64_t sid = S.sid[slot];
-    const int shower = S.shower[slot];
-    const double m = pid_mass(pid);
-    const double pme = fmax(fmax(min_calc(T, pid), T.d.min_energy), m);
-    double E = (double)p0.x;
-    float nx = p0.y, ny = p0.z, nz = p0.w;
-    {
-      const float pn = sqrtf(nx * nx + ny * ny + nz * nz);
-      if (pn > 0.0f) {
-        nx /= pn;
-        ny /= pn;
-        nz /= pn;
+  while (true) {
+    const unsigned long long am = __ballot(act);
+    // ---- finish parked particles in batches: write pf/rf, choose the process, queue the draw
+    const unsigned long long wm = __ballot(wait);
+    if (wm && (__popcll(wm) >= 16 || am == 0ull)) {
+      int want = -1;
+      if (wait) {
+        steps += k;
+        S.pf[slot] = make_float4((float)E, (float)(p * nx), (float)(p * ny), (float)(p * nz));
+        S.rf[slot] = make_float4(rx, ry, rz, (float)k);
+        // process choice (shower.py:665-698) behind sample_scattering's threshold (469-470)
+        if (E > pme) {
+          int ch[3];
+          const int nch = channels(pid, ch);
+          double wv[3], SC = 0.0;
+          for (int c = 0; c < nch; ++c) {
+            wv[c] = L.nsigma(ch[c], E);
+            SC += wv[c];
+          }
+          if (nch > 0 && SC > 0.0 && !isnan(SC)) {
+            const double uu = u01d(draw_block(seed, sid, 0, PURPOSE_INTERACT).x) * SC;
+            double c = 0.0;
+            int pick = nch - 1;
+            for (int qq = 0; qq < nch; ++qq) {
+              c += wv[qq];
+              if (uu < c) {
+                pick = qq;
+                break;
+              }
+            }
+            want = ch[pick];
+          }
+        }
+        if (shower_energy) {
+          unsafeAtomicAdd(shower_energy + 4 * shower + 0, dep);
+          if (want < 0) unsafeAtomicAdd(shower_energy + 4 * shower + 1, E);
+        }
+        wait = false;
+      }
+      // wave-aggregated appends to the per-process request regions (SM processes)
+      if (__ballot(want >= 0)) {
+        for (int pp = 0; pp < 8; ++pp) {
+          const unsigned long long mask = __ballot(want == pp);
+          if (!mask) continue;
+          const int leader = __ffsll((long long)mask) - 1;
+          unsigned base = 0;
+          if (lane == leader) base = atomicAdd(R.count + pp, (unsigned)__popcll(mask));
+          base = __shfl(base, leader);
+          if (want == pp) {
+            const unsigned idx = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+            R.slot[(int64_t)pp * R.cap + idx] = (int32_t)slot;
+            R.E[(int64_t)pp * R.cap + idx] = E;
+          }
+        }
       }
     }
-    double p = sqrt(fmax(E * E - m * m, 0.0));
-    float rx = r0.x, ry = r0.y, rz = r0.z;
-    double dep = 0.0;
-    if (E < pme) {
-      // ends where it starts (shower.py:532-536)
-    } else if (pid == 22) {
-      const double mfp = mean_free_path(T, 22, E);
-      const U4 u = draw_block(seed, sid, 0, PURPOSE_STEP);
-      const double dist = -mfp * log1p(-u01d(u.x));
-      rx += (float)(nx * dist);
-      ry += (float)(ny * dist);
-      rz += (float)(nz * dist);
-      steps = 1;
-    } else {
-      const double dEdx = T.d.dEdx;
-      uint32_t k = 0;
-      bool hard = false;
-      double dz = 0.0;
-      U4 u;
-      do {
-        const double mfp = mean_free_path(T, pid, E);
-        u = draw_block(seed, sid, k, PURPOSE_STEP);
-        dz = mfp / (6.0 + 14.0 * u01d(u.y));
-        if (u01d(u.x) > exp(-dz / mfp)) {
-          hard = true;
-        } else {
-          const double Eb = E;
-          E = E - dEdx * dz;
-          if (E <= m) E = m;
-          p = sqrt(E * E - m * m);
-          dep += Eb - E;
-          if (p > 0.0) {
-            rx += (float)(nx * dz);
-            ry += (float)(ny * dz);
-            rz += (float)(nz * dz);
-            if (T.d.MS) mcs_rotate(T, nx, ny, nz, E, p, m, m, dz, draw_block(seed, sid, k, PURPOSE_MCS));
+    // ---- refill empty lanes in batches
+    const bool idle = !act && !wait && !done;
+    const unsigned long long im = __ballot(idle);
+    if (im && (__popcll(im) >= 16 || am == 0ull)) {
+      const int64_t r = q.take(im, idle, counters + C_TRANSPORT);
+      if (idle) {
+        if (r < n) {
+          slot = first + r;
+          const float4 p0 = S.p0[slot];
+          const float4 r0 = S.r0[slot];
+          pid = S.pid[slot];
+          sid = S.sid[slot];
+          shower = S.shower[slot];
+          m = pid_mass(pid);
+          pme = fmax(fmax(min_calc(T, pid), T.d->min_energy), m);
+          E = (double)p0.x;
+          nx = p0.y;
+          ny = p0.z;
+          nz = p0.w;
+          const float pn2 = nx * nx + ny * ny + nz * nz;
+          if (pn2 > 0.0f) {
+            const float inv = rsqrtf(pn2);
+            nx *= inv;
+            ny *= inv;
+            nz *= inv;
+          }
+          p = sqrt(fmax(E * E - m * m, 0.0));
+          rx = r0.x;
+          ry = r0.y;
+          rz = r0.z;
+          dep = 0.0;
+          k = 0;
+          if (E < pme) {
+            wait = true;  // ends where it starts (shower.py:532-536)
+          } else if (pid == 22) {
+            // one exponential flight, no MCS (shower.py:538-553)
+            const double mfp = L.mfp(22, E);
+            const double dist = -mfp * (double)log1pf(-u01f(draw_block(seed, sid, 0, PURPOSE_STEP).x));
+            rx += (float)(nx * dist);
+            ry += (float)(ny * dist);
+            rz += (float)(nz * dist);
+            k = 1;
+            wait = true;
           } else {
-            nx = ny = nz = 0.0f;
+            act = true;
           }
+        } else {
+          done = true;
         }
+      }
+    }
+    if (__ballot(act) == 0ull) {
+      if (__ballot(wait || !done) == 0ull) break;
+      continue;
+    }
+    if (act) {
+      // one substep (shower.py:559-581); the hard test uses the substep's start mfp.
+      // exp in float32: the comparison flips only when u0 lies within ~1e-7 of the threshold
+      const double mfp = L.mfp(pid, E);
+      const U4 u = draw_block(seed, sid, k, PURPOSE_STEP);
+      const double dz = mfp / (6.0 + 14.0 * u01d(u.y));
+      const bool hard = u01d(u.x) > (double)expf(-(float)(dz / mfp));
+      if (!hard) charged_advance(T, dz, m, m, u.z, u.w, E, p, nx, ny, nz, rx, ry, rz, dep);
+      ++k;
+      if (hard || E < pme) {
+        // closing increment (shower.py:583-598); the reference omits m_lepton in its MCS call
+        const U4 c = draw_block(seed, sid, 0, PURPOSE_CLOSE);
+        const double distC = u01d(c.x);
+        double dist;
+        if (E < pme) {
+          dist = distC * dz;
+        } else {
+          const double mfp2 = L.mfp(pid, E);
+          dist = -mfp2 * (double)log1pf(expm1f(-(float)(dz / mfp2)) * (float)distC);
+        }
+        charged_advance(T, dist, m, kMe, c.y, c.z, E, p, nx, ny, nz, rx, ry, rz, dep);
         ++k;
-      } while (!hard && E >= pme);
-      const double distC = u01d(u.z);
-      double last;
-      if (E < pme) {
-        last = distC * dz;
-      } else {
-        const double mfp = mean_free_path(T, pid, E);
-        last = -mfp * log1p(expm1(-dz / mfp) * distC);
+        act = false;
+        wait = true;
       }
-      const double Eb = E;
-      E = E - dEdx * last;
-      if (E <= m) E = m;
-      p = sqrt(E * E - m * m);
-      dep += Eb - E;
-      if (p > 0.0) {
-        rx += (float)(nx * last);
-        ry += (float)(ny * last);
-        rz += (float)(nz * last);
-        if (T.d.MS) mcs_rotate(T, nx, ny, nz, E, p, m, kMe, last, draw_block(seed, sid, 0, PURPOSE_MCS_CLOSE));
-      } else {
-        nx = ny = nz = 0.0f;
-      }
-      steps = k + 1;
-    }
-    S.pf[slot] = make_float4((float)E, (float)(p * nx), (float)(p * ny), (float)(p * nz));
-    S.rf[slot] = make_float4(rx, ry, rz, (float)steps);
-    // process choice (shower.py:665-698) and the threshold of sample_scattering (469-470)
-    bool interacts = false;
-    if (E > pme) {
-      int ch[3];
-      const int nch = channels(pid, ch);
-      double w[3], SC = 0.0;
-      for (int c = 0; c < nch; ++c) {
-        w[c] = nsigma(T, ch[c], E);
-        SC += w[c];
-      }
-      if (nch > 0 && SC > 0.0 && !isnan(SC)) {
-        const double uu = u01d(draw_block(seed, sid, 0, PURPOSE_INTERACT).x) * SC;
-        double c = 0.0;
-        int pick = nch - 1;
-        for (int q = 0; q < nch; ++q) {
-          c += w[q];
-          if (uu < c) {
-            pick = q;
-            break;
-          }
-        }
-        want = ch[pick];
-        E_req = E;
-        interacts = true;
-      }
-    }
-    if (shower_energy) {
-      atomicAdd(shower_energy + 4 * shower + 0, dep);
-      if (!interacts) atomicAdd(shower_energy + 4 * shower + 1, E);
     }
   }
-  if (stats) wave_stat(stats, ST_STEPS, steps);
-  // wave-aggregated appends to the per-process request lists
-  for (int p = 0; p < PETITE_NPROC; ++p) {
-    const unsigned long long mask = __ballot(want == p);
-    if (!mask) continue;
-    const int leader = __ffsll((long long)mask) - 1;
-    unsigned base = 0;
-    if (lane == leader) base = atomicAdd(R.count + p, (unsigned)__popcll(mask));
-    base = __shfl(base, leader);
-    if (want == p) {
-      const unsigned idx = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
-      R.slot[(int64_t)p * R.cap + idx] = (int32_t)slot;
-      R.E[(int64_t)p * R.cap + idx] = E_req;
-    }
-  }
+  wave_stat(counters, C_STATS + ST_STEPS, steps);
 }
 
 // -------------------------------------------------------------- emission --
 // sample_scattering's kinematics, lab rotation and child bookkeeping
 // (shower.py:467-507, 702-706) for every finished draw.
-__global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, DrawIO io, uint64_t seed,
+struct EmitLaunch {
+  const int32_t* req_slot;
+  const double* req_E;
+  int64_t req_cap;
+  int64_t seg[PETITE_NPROC + 1];  // prefix sums of the per-process counts
+  int64_t total;
+  const float* x_out;
+  const int32_t* trials_out;
+};
+
+__global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunch io, uint64_t seed,
                                               unsigned long long* store_count, double* shower_energy,
                                               unsigned long long* stats) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -470,7 +662,7 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, DrawIO io
     const int shower = S.shower[slot];
     const int32_t tr = io.trials_out[oidx];
     if (tr <= 0) {
-      if (shower_energy) atomicAdd(shower_energy + 4 * shower + 1, E);
+      if (shower_energy) unsafeAtomicAdd(shower_energy + 4 * shower + 1, E);
     } else {
       const float* xs = io.x_out + 4 * oidx;
       const uint64_t sid = S.sid[slot];
@@ -499,11 +691,11 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, DrawIO io
         double ox, oy, oz;
         frame_apply_d(nx, ny, nz, in[k], ox, oy, oz);
         lab[k] = V4{in[k].e, ox, oy, oz};
-        keep[k] = lab[k].e > T.d.min_energy;
-        if (!keep[k] && shower_energy) atomicAdd(shower_energy + 4 * shower + 2, lab[k].e);
+        keep[k] = lab[k].e > T.d->min_energy;
+        if (!keep[k] && shower_energy) unsafeAtomicAdd(shower_energy + 4 * shower + 2, lab[k].e);
       }
       if (shower_energy && (p == COMP || p == ANN || p == MOLLER || p == BHABHA || p == MUONE))
-        atomicAdd(shower_energy + 4 * shower + 3, 1.0);
+        unsafeAtomicAdd(shower_energy + 4 * shower + 3, 1.0);
       if (stats) {
         const unsigned long long nd = (!keep[0]) + (!keep[1]);
         if (nd) atomicAdd(stats + ST_DROPPED, nd);
@@ -558,6 +750,8 @@ struct PetiteCtx {
   double* d64 = nullptr;
   float* d32 = nullptr;
   double* dmaxf = nullptr;
+  PetiteTableDesc* ddesc = nullptr;
+  double* dsig_lds = nullptr;
   int32_t maxE = 0;
   DevTables dev;
   // scratch
@@ -568,10 +762,11 @@ struct PetiteCtx {
   float* res_w = nullptr;
   int64_t req_cap = 0;
   unsigned int* req_count = nullptr;
-  unsigned long long* counters = nullptr;  // [0] refill, [1] store count, [8..8+ST_N) stats
-  PetitePrimary* d_prim = nullptr;
+  unsigned long long* counters = nullptr;  // C_N entries, layout CounterIdx
   bool timing = false;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  int n_cu = 256;
+  double tbar[PETITE_NPROC];  // running mean of candidates per draw, per process
 };
 
 static int ensure_requests(PetiteCtx* c, int64_t cap) {
@@ -584,6 +779,8 @@ static int ensure_requests(PetiteCtx* c, int64_t cap) {
     (void)hipFree(c->res_x);
     (void)hipFree(c->res_trials);
     (void)hipFree(c->res_w);
+    c->req_slot = nullptr;
+    c->req_cap = 0;
   }
   const int64_t n = cap * PETITE_NPROC;
   HIP_TRY(hipMalloc(&c->req_slot, n * sizeof(int32_t)));
@@ -592,6 +789,44 @@ static int ensure_requests(PetiteCtx* c, int64_t cap) {
   HIP_TRY(hipMalloc(&c->res_trials, n * sizeof(int32_t)));
   HIP_TRY(hipMalloc(&c->res_w, n * sizeof(float)));
   c->req_cap = cap;
+  return PETITE_OK;
+}
+
+// Launch plan of the per-process draw kernels (they run one after the other
+// on the stream, each with the whole GPU): one lane per request while there
+// are at least four requests per lane; below that, groups of G lanes share
+// a request, sized so the GPU stays filled and each group sees a few requests.
+static void plan_draw(PetiteCtx* c, const unsigned int* rq, DrawLaunch& io, int64_t& blocks_total) {
+  const int full_blocks = c->n_cu * 4;  // 4-wave blocks at 4 waves per SIMD
+  const double full_lanes = 256.0 * full_blocks;
+  io.block_start[0] = 0;
+  for (int p = 0; p < PETITE_NPROC; ++p) {
+    int b = 0, lg = 0;
+    io.count[p] = (int32_t)rq[p];
+    const double cnt = (double)rq[p];
+    if (rq[p]) {
+      if (cnt >= 4.0 * full_lanes) {
+        b = full_blocks;
+      } else {
+        const double want = 4.0 * full_lanes / cnt;
+        while ((1 << lg) < want && lg < 6) ++lg;
+        const double lanes = std::min(full_lanes, cnt * (double)(1 << lg));
+        b = std::max(1, (int)std::ceil(lanes / 256.0));
+      }
+    }
+    io.lgG[p] = lg;
+    io.block_start[p + 1] = io.block_start[p] + b;
+  }
+  blocks_total = io.block_start[PETITE_NPROC];
+}
+
+static int launch_draws(PetiteCtx* c, const DrawLaunch& io, uint64_t seed, hipStream_t s) {
+  for (int p = 0; p < PETITE_NPROC; ++p) {
+    const int b = io.block_start[p + 1] - io.block_start[p];
+    if (b <= 0 || io.count[p] <= 0 || c->desc.grid[p].n_E <= 0) continue;
+    hipLaunchKernelGGL(kDrawKernels[p], dim3((unsigned)b), dim3(256), 0, s, c->dev, io, seed);
+    HIP_TRY(hipGetLastError());
+  }
   return PETITE_OK;
 }
 
@@ -631,18 +866,20 @@ int petite_ctx_create(const PetiteTableDesc* desc, const double* blob64, int64_t
   if (hipMalloc(&c->d64, n64 * sizeof(double)) != hipSuccess || hipMalloc(&c->d32, n32 * sizeof(float)) != hipSuccess ||
       hipMalloc(&c->dmaxf, (size_t)PETITE_NPROC * maxE * sizeof(double)) != hipSuccess ||
       hipMalloc(&c->req_count, PETITE_NPROC * sizeof(unsigned int)) != hipSuccess ||
-      hipMalloc(&c->counters, 32 * sizeof(unsigned long long)) != hipSuccess)
+      hipMalloc(&c->counters, C_N * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&c->ddesc, sizeof(PetiteTableDesc)) != hipSuccess)
     return cleanup(fail(PETITE_ERR_HIP, "petite_ctx_create: hipMalloc failed"));
   if (hipMemcpy(c->d64, blob64, n64 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(c->d32, blob32, n32 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(c->dmaxf, 0, (size_t)PETITE_NPROC * maxE * sizeof(double)) != hipSuccess)
+      hipMemset(c->dmaxf, 0, (size_t)PETITE_NPROC * maxE * sizeof(double)) != hipSuccess ||
+      hipMemcpy(c->ddesc, desc, sizeof(PetiteTableDesc), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(fail(PETITE_ERR_HIP, "petite_ctx_create: upload failed"));
   DevTables& T = c->dev;
   T.f64 = c->d64;
   T.f32 = c->d32;
   T.maxf = c->dmaxf;
   T.maxE = maxE;
-  T.d = *desc;
+  T.d = c->ddesc;
   const double Z = desc->Z;
   const double a0 = 184.15 * pow(2.718, -0.5) * pow(Z, -1.0 / 3.0) / kMe;
   const double a04 = a0 * a0 * a0 * a0;
@@ -651,6 +888,32 @@ int petite_ctx_create(const PetiteTableDesc* desc, const double* blob64, int64_t
   T.brem_C_e = (float)Ce;
   T.brem_C_mu = (float)Cmu;
   T.a0sq = (float)(a0 * a0);
+  for (int p = 0; p < PETITE_NPROC; ++p) c->tbar[p] = 16.0;
+  T.mcs_cA = (float)(0.157 * Z * (Z + 1.0) / desc->A);
+  T.mcs_cB = (float)(2.007e-5 * pow(Z, 2.0 / 3.0));
+  T.mcs_cC = (float)(3.34 * (Z * kAlpha) * (Z * kAlpha));
+  {
+    // n*sigma tables of the transport kernel, laid out for LDS
+    std::vector<double> lds(4 * PETITE_NPROC, 0.0);
+    for (int p = 0; p < PETITE_NPROC; ++p) {
+      const PetiteSigmaDesc& sd = desc->sigma[p];
+      lds[4 * p + 0] = sd.lnE0;
+      lds[4 * p + 1] = sd.inv_dlog;
+      lds[4 * p + 3] = sd.n;
+      if (sd.n <= 0) continue;
+      lds[4 * p + 2] = (double)lds.size();
+      for (int i = 0; i < sd.n; ++i) lds.push_back(blob64[sd.E_off + i]);
+      for (int i = 0; i < sd.n; ++i) lds.push_back(blob64[sd.v_off + i]);
+    }
+    T.lds_doubles = (int32_t)lds.size();
+    if (hipMalloc(&c->dsig_lds, lds.size() * sizeof(double)) != hipSuccess ||
+        hipMemcpy(c->dsig_lds, lds.data(), lds.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(PETITE_ERR_HIP, "petite_ctx_create: LDS table upload failed"));
+  }
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+    c->n_cu = prop.multiProcessorCount;
   *out = c;
   return PETITE_OK;
 }
@@ -661,7 +924,8 @@ void petite_ctx_destroy(PetiteCtx* c) {
   for (hipEvent_t& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   for (void* p : {(void*)c->d64, (void*)c->d32, (void*)c->dmaxf, (void*)c->req_slot, (void*)c->req_E, (void*)c->res_x,
-                  (void*)c->res_trials, (void*)c->res_w, (void*)c->req_count, (void*)c->counters, (void*)c->d_prim})
+                  (void*)c->res_trials, (void*)c->res_w, (void*)c->req_count, (void*)c->counters, (void*)c->ddesc,
+                  (void*)c->dsig_lds})
     if (p) (void)hipFree(p);
   delete c;
 }
@@ -705,24 +969,34 @@ int petite_vegas_draw(PetiteCtx* c, const int32_t* proc, const double* E, const 
   if (!c || n < 0 || (n > 0 && (!proc || !E || !sid || !x_out || !trials_out || !w_out)))
     return fail(PETITE_ERR_ARG, "petite_vegas_draw: bad arguments");
   if (n == 0) return PETITE_OK;
+  if (n > INT32_MAX) return fail(PETITE_ERR_ARG, "petite_vegas_draw: more than 2^31-1 requests");
   hipStream_t s = (hipStream_t)stream;
-  DrawIO io{};
-  io.proc = proc;
-  io.E = E;
-  io.sid = sid;
+  int rc = ensure_requests(c, n);
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(c->counters, 0, C_N * sizeof(unsigned long long), s));
+  HIP_TRY(hipMemsetAsync(c->req_count, 0, PETITE_NPROC * sizeof(unsigned int), s));
+  hipLaunchKernelGGL(k_bucket, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c->dev, proc, E, n, c->req_slot, c->req_E,
+                     c->req_cap, c->req_count, trials_out, w_out);
+  HIP_TRY(hipGetLastError());
+  unsigned int rq[PETITE_NPROC];
+  HIP_TRY(hipMemcpyAsync(rq, c->req_count, sizeof(rq), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  DrawLaunch io{};
+  io.req_slot = c->req_slot;
+  io.req_E = c->req_E;
+  io.sid_src = sid;
+  io.req_cap = c->req_cap;
+  io.explicit_out = 1;
   io.x_out = x_out;
   io.trials_out = trials_out;
   io.w_out = w_out;
-  io.total = n;
-  io.counter = c->counters;
-  io.stats = c->counters + 8;
-  HIP_TRY(hipMemsetAsync(c->counters, 0, 32 * sizeof(unsigned long long), s));
-  const int64_t waves = std::min<int64_t>((n + 63) / 64, 256 * 16);
-  const int blocks = (int)((waves + 3) / 4);
-  hipLaunchKernelGGL(k_draw<false>, dim3(blocks), dim3(256), 0, s, c->dev, io, seed);
-  HIP_TRY(hipGetLastError());
+  io.counters = c->counters;
+  int64_t blocks = 0;
+  plan_draw(c, rq, io, blocks);
+  rc = launch_draws(c, io, seed, s);
+  if (rc) return rc;
   unsigned long long st[ST_N];
-  HIP_TRY(hipMemcpyAsync(st, c->counters + 8, sizeof(st), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(st, c->counters + C_STATS, sizeof(st), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (st[ST_NOSAMPLE]) return fail(PETITE_ERR_NO_SAMPLE, "petite_vegas_draw: No Sample Found within max_trials");
   return PETITE_OK;
@@ -748,65 +1022,78 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   hipStream_t s = (hipStream_t)stream;
   DevStore S{(float4*)store->p0, (float4*)store->r0, (float4*)store->pf, (float4*)store->rf, store->pid, store->parent,
              store->meta, store->shower, store->sid, store->capacity};
-  unsigned long long* refill = c->counters;
-  unsigned long long* scount = c->counters + 1;
-  unsigned long long* st = c->counters + 8;
-  HIP_TRY(hipMemsetAsync(c->counters, 0, 32 * sizeof(unsigned long long), s));
+  unsigned long long* ctr = c->counters;
+  HIP_TRY(hipMemsetAsync(ctr, 0, C_N * sizeof(unsigned long long), s));
   const unsigned long long n0 = (unsigned long long)n;
-  HIP_TRY(hipMemcpyAsync(scount, &n0, sizeof(n0), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(ctr + C_STORE, &n0, sizeof(n0), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_init_primaries, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, prim, n);
   HIP_TRY(hipGetLastError());
   int64_t first = 0, count = n, written = n, iters = 0;
   double t_tr = 0.0, t_dr = 0.0, t_em = 0.0;
   int64_t l_tr = 0, l_dr = 0, l_em = 0;
   std::memset(stats, 0, sizeof(*stats));
+  const int64_t max_waves = (int64_t)c->n_cu * 20;
+  struct {
+    unsigned int rq[PETITE_NPROC];
+  } hrq;
+  unsigned long long ptr[2 * PETITE_NPROC];
   while (count > 0) {
     int rc = ensure_requests(c, count);
     if (rc) return rc;
     Requests R{c->req_slot, c->req_E, c->req_cap, c->req_count};
     HIP_TRY(hipMemsetAsync(c->req_count, 0, PETITE_NPROC * sizeof(unsigned int), s));
+    HIP_TRY(hipMemsetAsync(ctr, 0, 17 * sizeof(unsigned long long), s));  // refill counters
+    const int64_t tw = std::min<int64_t>((count + 63) / 64, max_waves);
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], s));
-    hipLaunchKernelGGL(k_transport, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, c->dev, S, first, count, seed,
-                       R, shower_energy, st);
+    hipLaunchKernelGGL(k_transport, dim3((unsigned)((tw + 3) / 4)), dim3(256), c->dev.lds_doubles * sizeof(double), s,
+                       c->dev, S, first, count, seed, R, shower_energy, ctr, c->dsig_lds);
     HIP_TRY(hipGetLastError());
     if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], s));
-    unsigned int rq[PETITE_NPROC];
-    HIP_TRY(hipMemcpyAsync(rq, c->req_count, sizeof(rq), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(hrq.rq, c->req_count, sizeof(hrq.rq), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(ptr, ctr + C_PTRIALS, sizeof(ptr), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    DrawIO io{};
+    for (int p = 0; p < PETITE_NPROC; ++p)
+      if (ptr[PETITE_NPROC + p] > 0) c->tbar[p] = (double)ptr[p] / (double)ptr[PETITE_NPROC + p];
+    DrawLaunch io{};
     io.req_slot = c->req_slot;
     io.req_E = c->req_E;
-    io.store_sid = store->sid;
+    io.sid_src = store->sid;
     io.req_cap = c->req_cap;
-    io.seg[0] = 0;
-    for (int p = 0; p < PETITE_NPROC; ++p) io.seg[p + 1] = io.seg[p] + rq[p];
-    io.total = io.seg[PETITE_NPROC];
+    io.explicit_out = 0;
     io.x_out = c->res_x;
     io.trials_out = c->res_trials;
     io.w_out = c->res_w;
-    io.counter = refill;
-    io.stats = st;
-    if (io.total > 0) {
-      HIP_TRY(hipMemsetAsync(refill, 0, sizeof(unsigned long long), s));
-      const int64_t waves = std::min<int64_t>((io.total + 63) / 64, 256 * 16);
+    io.counters = ctr;
+    int64_t blocks = 0;
+    plan_draw(c, hrq.rq, io, blocks);
+    EmitLaunch em{};
+    em.req_slot = c->req_slot;
+    em.req_E = c->req_E;
+    em.req_cap = c->req_cap;
+    em.seg[0] = 0;
+    for (int p = 0; p < PETITE_NPROC; ++p) em.seg[p + 1] = em.seg[p] + hrq.rq[p];
+    em.total = em.seg[PETITE_NPROC];
+    em.x_out = c->res_x;
+    em.trials_out = c->res_trials;
+    if (em.total > 0) {
       if (c->timing) HIP_TRY(hipEventRecord(c->ev[2], s));
-      hipLaunchKernelGGL(k_draw<true>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, c->dev, io, seed);
-      HIP_TRY(hipGetLastError());
+      rc = launch_draws(c, io, seed, s);
+      if (rc) return rc;
       if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], s));
-      hipLaunchKernelGGL(k_emit, dim3((unsigned)((io.total + 255) / 256)), dim3(256), 0, s, c->dev, S, io, seed, scount,
-                         shower_energy, st);
+      hipLaunchKernelGGL(k_emit, dim3((unsigned)((em.total + 255) / 256)), dim3(256), 0, s, c->dev, S, em, seed,
+                         ctr + C_STORE, shower_energy, ctr + C_STATS);
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->ev[4], s));
     }
     unsigned long long now = 0;
-    HIP_TRY(hipMemcpyAsync(&now, scount, sizeof(now), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&now, ctr + C_STORE, sizeof(now), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (c->timing) {
       float ms = 0.0f;
       HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
       t_tr += ms;
       ++l_tr;
-      if (io.total > 0) {
+      if (em.total > 0) {
         HIP_TRY(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
         t_dr += ms;
         ++l_dr;
@@ -818,14 +1105,11 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
     ++iters;
     first = written;
     count = (int64_t)now - written;
-    if ((int64_t)now > store->capacity) {
-      written = (int64_t)now;
-      break;
-    }
     written = (int64_t)now;
+    if (written > store->capacity) break;
   }
   unsigned long long hs[ST_N];
-  HIP_TRY(hipMemcpyAsync(hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(hs, ctr + C_STATS, sizeof(hs), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   stats->n_particles = written;
   stats->n_steps = (int64_t)hs[ST_STEPS];

@@ -11,9 +11,8 @@
 namespace petite {
 
 enum Purpose : uint32_t {
-  PURPOSE_STEP = 0,        // substep k: u0 hard test, u1 -> U(6,20), u2 closing fraction
-  PURPOSE_MCS = 1,         // MCS after substep k: u0 radius, u1 sign, u2 azimuth
-  PURPOSE_MCS_CLOSE = 2,   // MCS of the closing increment
+  PURPOSE_STEP = 0,        // substep k: x hard test, y -> U(6,20), z MCS radius, w MCS azimuth (sign: bit 0 of w)
+  PURPOSE_CLOSE = 2,       // closing increment: x fraction, y MCS radius, z MCS azimuth (sign: bit 0 of z)
   PURPOSE_INTERACT = 3,    // u0 process choice, u1 kinematic azimuth, u2/u3 extra
   PURPOSE_VEGAS_A = 4,     // candidate j: y0..y3 (ndim <= 3: u3 = accept uniform)
   PURPOSE_VEGAS_B = 5,     // candidate j (ndim 4): u0 = accept uniform

@@ -36,27 +36,48 @@ struct DevTables {
   float brem_C_e;      // 8/pi alpha (alpha/m)^2 m^4 Z^2 a0^4 for m = m_e
   float brem_C_mu;     // same for m = m_mu (a0 keeps m_e, all_processes.py:92-103)
   float a0sq;          // a0^2 [GeV^-2]
-  float pad;
-  PetiteTableDesc d;
+  float mcs_cA;        // Lynch-Dahl constants of the material: 0.157 Z (Z+1) / A,
+  float mcs_cB;        //   2.007e-5 Z^(2/3),
+  float mcs_cC;        //   3.34 (Z alpha)^2  (moliere.py:196-219)
+  int32_t lds_doubles; // size of the n*sigma tables staged in LDS by the transport kernel
+  const PetiteTableDesc* d;  // device copy of the descriptor (read through the scalar cache)
 };
 
 // ---------------------------------------------------------------- tables ---
 
+// Interval index lo (0 <= lo <= n-2) with xe[lo] <= E, the largest such
+// (E inside [xe[0], xe[n-1]]).  Geometric grids (inv_dlog > 0) take an O(1)
+// guess from ln E and correct it against the stored nodes; others bisect.
+__device__ inline int grid_interval(const double* xe, int n, double lnE0, double inv_dlog, double E) {
+  int lo;
+  if (inv_dlog > 0.0) {
+    lo = (int)((logf((float)E) - (float)lnE0) * (float)inv_dlog);
+    lo = lo < 0 ? 0 : (lo > n - 2 ? n - 2 : lo);
+    while (lo > 0 && xe[lo] > E) --lo;
+    while (lo < n - 2 && xe[lo + 1] <= E) ++lo;
+  } else {
+    lo = 0;
+    int hi = n - 1;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (xe[mid] <= E) lo = mid; else hi = mid;
+    }
+  }
+  return lo;
+}
+
 // Linear interpolation with zero outside [E0, E_{n-1}] (scipy interp1d with
 // fill_value=0, bounds_error=False as in src/PETITE/shower.py:280-295).
 __device__ inline double nsigma(const DevTables& T, int proc, double E) {
-  const PetiteSigmaDesc& s = T.d.sigma[proc];
+  const PetiteSigmaDesc& s = T.d->sigma[proc];
   if (s.n <= 0) return 0.0;
   const double* xe = T.f64 + s.E_off;
   const double* ye = T.f64 + s.v_off;
   if (!(E >= xe[0]) || E > xe[s.n - 1]) return 0.0;
-  int lo = 0, hi = s.n - 1;  // xe[lo] <= E <= xe[hi]
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (xe[mid] <= E) lo = mid; else hi = mid;
-  }
-  const double t = (E - xe[lo]) / (xe[hi] - xe[lo]);
-  return ye[lo] + t * (ye[hi] - ye[lo]);
+  if (s.n == 1) return ye[0];
+  const int lo = grid_interval(xe, s.n, s.lnE0, s.inv_dlog, E);
+  const double t = (E - xe[lo]) / (xe[lo + 1] - xe[lo]);
+  return ye[lo] + t * (ye[lo + 1] - ye[lo]);
 }
 
 // Channels per species in the reference's order (shower.py:665-698).
@@ -81,41 +102,63 @@ __device__ inline double mean_free_path(const DevTables& T, int pid, double E) {
 
 // LU_Key = argmin|E_i - E| + 1, clamped to n_E - 1 (shower.py:416-426).
 __device__ inline int lu_key(const DevTables& T, int proc, double E) {
-  const PetiteGridDesc& g = T.d.grid[proc];
+  const PetiteGridDesc& g = T.d->grid[proc];
   const double* Eg = T.f64 + g.E_off;
-  int lo = 0, hi = g.n_E - 1;
-  if (E <= Eg[0]) return g.n_E > 1 ? 1 : 0;
-  if (E >= Eg[hi]) return hi;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (Eg[mid] <= E) lo = mid; else hi = mid;
-  }
-  // argmin picks the first index on ties (numpy.argmin)
-  int best = (fabs(Eg[lo] - E) <= fabs(Eg[hi] - E)) ? lo : hi;
-  int k = best + 1;
-  return k > g.n_E - 1 ? g.n_E - 1 : k;
+  const int n = g.n_E;
+  if (n < 2) return 0;
+  if (E <= Eg[0]) return 1;
+  if (E >= Eg[n - 1]) return n - 1;
+  const int lo = grid_interval(Eg, n, g.lnE0, g.inv_dlog, E);
+  // argmin takes the first index on ties (numpy.argmin)
+  const int best = (fabs(Eg[lo] - E) <= fabs(Eg[lo + 1] - E)) ? lo : lo + 1;
+  const int k = best + 1;
+  return k > n - 1 ? n - 1 : k;
 }
 
 // VEGAS map transform with an integer-exact increment index: the uniform
 // y = (2m+1) 2^-24 (m = the top 23 bits of the Philox word) times N is formed
 // as an exact 34-bit integer, so the bin and the fraction equal what float64
-// gives on the host.
-__device__ inline double map_point(const DevTables& T, int proc, int ie, const uint32_t* r, int ndim, double* x) {
-  const PetiteGridDesc& g = T.d.grid[proc];
-  const float* base = T.f32 + g.node_off + (int64_t)ie * g.stride;
-  double J = 1.0;
-  for (int d = 0; d < ndim; ++d) {
-    const uint32_t N = (uint32_t)g.ninc[d];
+// gives on the host.  ``base`` points at the nodes of one grid energy; the
+// node pairs of all dimensions are gathered before they are used.
+template <int ND>
+__device__ inline double map_point_n(const float* base, const int32_t* ninc, const uint32_t* r, double* x) {
+  uint32_t idx[ND];
+  double fr[ND];
+  float g0[ND], g1[ND];
+  int64_t off = 0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const uint32_t N = (uint32_t)ninc[d];
     const uint64_t q = (uint64_t)(2u * (r[d] >> 9) + 1u) * (uint64_t)N;
-    const uint32_t i = (uint32_t)(q >> 24);
-    const double fr = (double)(uint32_t)(q & 0xFFFFFFu) * 0x1.0p-24;
-    const double g0 = (double)base[i];
-    const double dg = (double)base[i + 1] - g0;
-    x[d] = g0 + dg * fr;
-    J *= (double)N * dg;
-    base += N + 1;
+    idx[d] = (uint32_t)(q >> 24) + (uint32_t)off;
+    fr[d] = (double)(uint32_t)(q & 0xFFFFFFu) * 0x1.0p-24;
+    off += N + 1;
+  }
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    g0[d] = base[idx[d]];
+    g1[d] = base[idx[d] + 1];
+  }
+  double J = 1.0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) {
+    const double a = (double)g0[d];
+    const double dg = (double)g1[d] - a;
+    x[d] = a + dg * fr[d];
+    J *= (double)ninc[d] * dg;
   }
   return J;
+}
+
+__device__ inline double map_point(const DevTables& T, int proc, int ie, const uint32_t* r, int ndim, double* x) {
+  const PetiteGridDesc& g = T.d->grid[proc];
+  const float* base = T.f32 + g.node_off + (int64_t)ie * g.stride;
+  switch (ndim) {
+    case 1: return map_point_n<1>(base, g.ninc, r, x);
+    case 3: return map_point_n<3>(base, g.ninc, r, x);
+    case 4: return map_point_n<4>(base, g.ninc, r, x);
+    default: return map_point_n<2>(base, g.ninc, r, x);
+  }
 }
 
 // -------------------------------------------------------------- integrands --
@@ -135,7 +178,7 @@ __device__ inline float cospi_f(double t) {
 
 // Brem / MuonBrem, all_processes.py:139-206.  PF*FF = C epp d dp /(w ep (1+a0^2 q^2)^2).
 __device__ inline float f_brem(const DevTables& T, double ep, double m, float C, const double* x) {
-  const double egmin = T.d.Eg_min;
+  const double egmin = T.d->Eg_min;
   const double range = ep - m - egmin;
   const double w = egmin + x[0] * range;
   const double epp = ep - w;
@@ -372,14 +415,14 @@ __device__ inline double integrand(const DevTables& T, int proc, double E, const
     case MUONBREM: return (double)f_brem(T, E, kMmu, T.brem_C_mu, x);
     case PAIRPROD: return (double)f_pairprod(T, E, x);
     case COMP: return f_compton(E, 0.0, x[0]);
-    case ANN: return f_annihilation(E, 0.0, T.d.Eg_min, x[0]);
-    case MOLLER: return f_moller(E, T.d.Ee_min, x[0]);
-    case BHABHA: return f_bhabha(E, T.d.Ee_min, x[0]);
-    case MUONE: return f_muone(E, T.d.Ee_min, x[0]);
-    case DARKBREM: return f_dark_brem(E, kMe, T.d.mV, T.d.mT, T.d.Z, T.d.A, x);
-    case DARKMUONBREM: return f_dark_brem(E, kMmu, T.d.mV, T.d.mT, T.d.Z, T.d.A, x);
-    case DARKANN: return f_dark_ann(E, T.d.mV, x[0]);
-    case DARKCOMP: return f_compton(E, T.d.mV, x[0]);
+    case ANN: return f_annihilation(E, 0.0, T.d->Eg_min, x[0]);
+    case MOLLER: return f_moller(E, T.d->Ee_min, x[0]);
+    case BHABHA: return f_bhabha(E, T.d->Ee_min, x[0]);
+    case MUONE: return f_muone(E, T.d->Ee_min, x[0]);
+    case DARKBREM: return f_dark_brem(E, kMe, T.d->mV, T.d->mT, T.d->Z, T.d->A, x);
+    case DARKMUONBREM: return f_dark_brem(E, kMmu, T.d->mV, T.d->mT, T.d->Z, T.d->A, x);
+    case DARKANN: return f_dark_ann(E, T.d->mV, x[0]);
+    case DARKCOMP: return f_compton(E, T.d->mV, x[0]);
     default: return 0.0;
   }
 }

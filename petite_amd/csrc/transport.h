@@ -180,14 +180,14 @@ __device__ inline void frame_apply_d(double nx, double ny, double nz, const V4& 
 }
 
 // -------------------------------------------------------------- MCS -----
-// Lynch-Dahl Gaussian width (moliere.py:196-219, 265-284).  pbeta is the
-// reference's p*beta in MeV with p = m_lep beta gamma; ptot its p.
-__device__ inline float lynch_dahl_theta0(float t_gcm2, float pMeV, float beta, float A, float Z) {
+// Lynch-Dahl Gaussian width (moliere.py:196-219, 265-284) with the material
+// constants folded: chi_c^2 = cA t / (p beta)^2 and
+// v = chi_c^2 / (2 (1-F) chi_a^2) = cA t / (2 (1-F) cB (beta^2 + cC)).
+__device__ inline float lynch_dahl_theta0(float cA, float cB, float cC, float t_gcm2, float pMeV, float beta) {
   const float F = 0.98f;
-  const float chic2 = 0.157f * Z * (Z + 1.0f) * (t_gcm2 / A) / (pMeV * beta * pMeV * beta);
-  const float za = Z * (float)kAlpha / beta;
-  const float chia2 = 2.007e-5f * cbrtf(Z * Z) * (1.0f + 3.34f * za * za) / (pMeV * pMeV);
-  const float v = 0.5f * (chic2 / chia2) / (1.0f - F);
+  const float pb = pMeV * beta;
+  const float chic2 = cA * t_gcm2 / (pb * pb);
+  const float v = cA * t_gcm2 / (2.0f * (1.0f - F) * cB * (beta * beta + cC));
   float g;  // (1+v) ln(1+v)/v - 1
   if (v < 0.1f) {
     // sum_{n>=1} (-1)^{n+1} v^n / (n (n+1))

@@ -157,6 +157,17 @@ class MaterialTables:
         return np.where((E < xg[0]) | (E > xg[-1]), 0.0, np.interp(E, xg, yg))
 
 
+def geometric(E):
+    """(ln E0, (n-1)/ln(E_last/E0)) for a geometric grid, else (0, 0)."""
+    E = np.asarray(E, dtype=np.float64)
+    if len(E) < 2 or np.any(E <= 0):
+        return 0.0, 0.0
+    r = np.log(E[1:] / E[:-1])
+    if np.max(np.abs(r / r.mean() - 1.0)) > 1e-9:
+        return 0.0, 0.0
+    return float(np.log(E[0])), float((len(E) - 1) / np.log(E[-1] / E[0]))
+
+
 def build_blobs(mt, dark=None):
     """Pack MaterialTables (and optional dark tables) into (TableDesc, blob64, blob32)."""
     desc = _lib.TableDesc()
@@ -200,12 +211,14 @@ def build_blobs(mt, dark=None):
             raise ValueError(f"{P}: node blob size mismatch")
         g.E_off = put64(E)
         g.node_off = put32(nodes)
+        g.lnE0, g.inv_dlog = geometric(E)
     for P, (xg, yg) in sigmas.items():
         code = all_process_code[P]
         s = desc.sigma[code]
         s.n = len(xg)
         s.E_off = put64(xg)
         s.v_off = put64(yg)
+        s.lnE0, s.inv_dlog = geometric(xg)
     desc.Z, desc.A, desc.rho, desc.dEdx = mt.Z, mt.A, mt.rho, mt.dEdx
     desc.mT = mt.A  # event_info 'mT': A_T (shower.py:435, dark_shower.py:672)
     desc.Eg_min, desc.Ee_min = mt.Eg_min, mt.Ee_min

@@ -55,12 +55,13 @@ def test_mcs_preserves_p_and_rayleigh_law():
     p4 = np.array([1.0, 0.1, -0.2, math.sqrt(1 - m_electron ** 2 - 0.05)])
     t, A, Z = 0.5, 12, 6
     n = 4000
-    u = philox.block(3, np.arange(n, dtype=np.uint64), 0, philox.PURPOSE_MCS)
+    w = philox.words(3, np.arange(n, dtype=np.uint64), 0, philox.PURPOSE_STEP)
+    u = philox.to_uniform(w)
     beta = np.linalg.norm(p4[1:]) / p4[0]
     th0 = moliere.theta0(t, beta, A, Z, m_electron)
     ang = []
     for i in range(n):
-        q = moliere.scatter(p4, t, A, Z, 1.0, m_electron, u[i])
+        q = moliere.scatter(p4, t, A, Z, 1.0, m_electron, u[i, 2], u[i, 3], 1.0 if int(w[i, 3]) & 1 else -1.0)
         assert abs(np.linalg.norm(q[1:]) - np.linalg.norm(p4[1:])) < 1e-12
         c = np.dot(q[1:], p4[1:]) / np.dot(p4[1:], p4[1:])
         ang.append(math.acos(min(1.0, c)))
