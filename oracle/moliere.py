@@ -9,7 +9,7 @@ and the sign s = +1 when bit 0 of the azimuth word is set (words of the
 substep's or the closing increment's Philox block, see oracle.shower).
 
 Pinned by: Rayleigh law of the space angle (KS) and |p| preservation
-(tests/test_oracle_moliere.py).
+(tests/test_oracle_physics.py).
 """
 import math
 

@@ -1,0 +1,58 @@
+"""The C ABI: libpetite_hip.so loads without a GPU, exports every entry point
+include/petite_hip.h declares, and its ctypes structures match the C layout."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from petite_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "petite_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(petite_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 8
+    for n in names:
+        assert hasattr(lib, n), n
+    assert sorted(_lib.EXPORTED) == names
+
+
+def test_struct_layouts_match_c(tmp_path):
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "petite_hip.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(PetiteTableDesc), '
+                   'sizeof(PetitePrimary), sizeof(PetiteStore), sizeof(PetiteStats), offsetof(PetiteTableDesc, grid), '
+                   'sizeof(PetiteGridDesc), sizeof(PetiteSigmaDesc)); return 0;}\n')
+    exe = tmp_path / "sz"
+    try:
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    except (FileNotFoundError, subprocess.CalledProcessError):
+        pytest.skip("no C compiler")
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = [ctypes.sizeof(_lib.TableDesc), ctypes.sizeof(_lib.Primary), ctypes.sizeof(_lib.Store),
+            ctypes.sizeof(_lib.Stats), _lib.TableDesc.grid.offset, ctypes.sizeof(_lib.GridDesc),
+            ctypes.sizeof(_lib.SigmaDesc)]
+    assert got == want
+
+
+def test_error_string_without_gpu():
+    lib = _lib.load()
+    rc = lib.petite_ctx_create(None, None, 0, None, 0, None)
+    assert rc == _lib.PETITE_ERR_ARG
+    assert b"bad arguments" in lib.petite_last_error()
+
+
+def test_primary_dtype_matches_struct():
+    from petite_amd.batch import PRIMARY_DTYPE
+    assert PRIMARY_DTYPE.itemsize == ctypes.sizeof(_lib.Primary)
