@@ -1,0 +1,267 @@
+"""Dark-sector reweighting (oracle, float64).
+
+Restates DarkShower (src/PETITE/dark_shower.py): mass selection (158-176),
+dark cross sections and thresholds (219-245), bound-electron cross sections
+(254-292 with src/PETITE/atomic_annihilation.py and atomic_compton.py),
+log-log n*sigma (294-309), the survival factors (src/PETITE/shower.py:298-354),
+weight arrays and drate tables with scipy.integrate.quad (311-593),
+GetBSMWeights (595-647), draw_dark_sample (649-704), draw_pe_sample (706-719)
+and produce_bsm_particle (721-804).  Random numbers come from the dark stream
+of each (SM particle, dark process) pair (oracle.philox, PURPOSE_DARK).
+
+Pinned by: the bound-electron closed forms, evaluated in 60-digit arithmetic
+(the `m` namespace argument), against the large-b expansions the reference
+carries beside them, continuity at the expr1/expr2 branch point and the 1s
+momentum-tail integral of the tree level; the stored dark_weights.pkl /
+dark_drate.pkl at the level of the SM tables' MC noise (the caches were made
+from SM tables other than the shipped sm_xsec.pkl).  The weight/drate quad
+integrals here check the product's panel quadrature on the same tables
+(tests/test_dark_tables.py).  Sampling (draw_dark_sample, produce_bsm_particle)
+reuses the pinned VEGAS and kinematics restatements.
+"""
+import math
+
+import numpy as np
+from scipy.integrate import quad
+
+from . import philox, vegas
+from .constants import GeVsqcm2, alpha_em, cmtom, m_electron, m_muon
+from .integrands import INTEGRAND
+from .kinematics import compton_bound, l_to_lV, radiative_return
+from .moliere import scatter
+from .shower import lose_energy, rotation_matrix
+from .tables import Grid, interp0, load_pack
+
+DARK_CODE = {"DarkBrem": 8, "DarkAnn": 9, "DarkComp": 10, "DarkMuonBrem": 11}
+
+
+def closest_lesser_value(input_list, input_value, below="clamp"):
+    arr = np.asarray(input_list)
+    index = (np.abs(arr - input_value)).argmin()
+    if arr[index] <= input_value:
+        return float(arr[index])
+    if index == 0 and below == "clamp":
+        return float(arr[0])
+    return float(arr[index - 1])
+
+
+class _NumpyMath:
+    log, atan, atan2, pi = np.log, np.arctan, np.arctan2, np.pi
+
+
+def f_ab(a, b):
+    return 1 / ((a - b) ** 2 + 1) ** 3
+
+
+def fancy_integral(a, b, m=_NumpyMath, series=True):
+    """atomic_annihilation.py:10-68.  `m` supplies log/atan/atan2/pi (float64
+    numpy by default; an mpmath namespace evaluates the same expression in
+    extended precision); series=False skips the b >= 1000 expansion."""
+    A_ = 1 + a ** 2
+    B = (1 + a ** 2) ** 3
+    C = 5 + 3 * a ** 2
+    D = (1 + a ** 2) ** 2
+    E = -1 + 15 * a ** 2 + 12 * a ** 4
+    term1 = (1 + a ** 2) * (a * B * C - D * E * b + a * (1 + a ** 2) * (-23 + 38 * a ** 2 + 21 * a ** 4) * b ** 2
+                            + 4 * (5 + a ** 2 - 18 * a ** 4 - 6 * a ** 6) * b ** 3 + 3 * a * (3 + a ** 2) * (-1 + 7 * a ** 2) * b ** 4
+                            + (7 - 29 * a ** 2 - 12 * a ** 4) * b ** 5 + a * (7 + 3 * a ** 2) * b ** 6)
+    term2 = (3 * (1 + a ** 2) ** 6 - a * (1 + a ** 2) ** 3 * (3 + 26 * a ** 2 + 15 * a ** 4) * b
+             + 3 * (1 + a ** 2) ** 2 * (-7 + 21 * a ** 2 + 23 * a ** 4 + 11 * a ** 6) * b ** 2
+             - 3 * a * (1 + a ** 2) * (7 + 69 * a ** 2 + 45 * a ** 4 + 15 * a ** 6) * b ** 3
+             + (-11 + 132 * a ** 2 + 9 * a ** 4 * (38 + 5 * a ** 2 * (4 + a ** 2))) * b ** 4
+             - 3 * a * (11 + 73 * a ** 2 + 41 * a ** 4 + 11 * a ** 6) * b ** 5
+             + 3 * (-1 + 27 * a ** 2 + 17 * a ** 4 + 5 * a ** 6) * b ** 6 - a * (15 + 10 * a ** 2 + 3 * a ** 4) * b ** 7)
+    term3 = term2
+    term4 = ((1 + a ** 2) ** 3 + 6 * a * (1 + a ** 2) ** 2 * b - 3 * (1 + 6 * a ** 2 + 5 * a ** 4) * b ** 2
+             + 4 * a * (3 + 5 * a ** 2) * b ** 3 - 3 * (1 + 5 * a ** 2) * b ** 4 + 6 * a * b ** 5 - b ** 6)
+    numerator = (term1 + term2 * m.atan2(a, 1) + term3 * m.atan2(1 + a ** 2 - a * b, b)
+                 + 4 * b * (term4 * (m.log(1 + (a - b) ** 2) - 2 * m.log(b))))
+    denominator = 8 * (1 + a ** 2) ** 3 * (1 + (a - b) ** 2) ** 3 * b
+    if b < 1000 or not series:
+        return numerator / denominator
+    return (3081 / 280 - (3081 * a ** 2) / 40) / b ** 8 - (709 * a) / (35 * b ** 7) - 91 / (30 * b ** 6)
+
+
+def sigma_atomic(k, mV, Zeff):
+    """atomic_annihilation.py:70-104 (local alpha = 1/137, m_e = 0.511 MeV)."""
+    al, me = 1 / 137, 0.511e-3
+    Lambda = Zeff * al * me
+    a = me / Lambda
+    b = mV ** 2 / (2 * k * Lambda)
+    s0 = (4.0 * np.pi * al) * ((mV ** 2 + 2 * me ** 2) * (2 / 3 / Lambda / me) * (1 / k ** 2) * f_ab(a, b))
+    s = 2 * k * me + me ** 2
+    beta = 2 * al / np.pi * (np.log(s / me ** 2) - 1.0)
+    s1 = (4.0 * np.pi * al) * ((mV ** 2 + 2 * me ** 2) * (2 / 3 / Lambda / me) * (1 / k ** 2) * (beta / 2) * fancy_integral(a, b))
+    return s0 + s1
+
+
+def fancy_series(a, b):
+    """Large-b expansion used by fancy_integral (atomic_annihilation.py:66-67)."""
+    return (3081 / 280 - (3081 * a ** 2) / 40) / b ** 8 - (709 * a) / (35 * b ** 7) - 91 / (30 * b ** 6)
+
+
+def _expr1(a, b, m=_NumpyMath):
+    """atomic_compton.py:6-44."""
+    delta = (a - b) ** 2 + 1
+    log_term = m.log(delta) - 2 * m.log(b)
+    num_part1 = 8 * b * (a ** 2 - 6 * a * b + 1) * log_term
+    inner = (-a * (a ** 2 + 1) ** 3 * (3 * a ** 2 + 5) + 6 * (a ** 4 + 4 * a ** 2 - 5) * b ** 5
+             - 4 * a * (6 * a ** 4 + 23 * a ** 2 - 31) * b ** 4 + (a - 1) * (a + 1) * (39 * a ** 4 + 190 * a ** 2 + 55) * b ** 3
+             - a * (a ** 2 + 1) * (33 * a ** 4 + 98 * a ** 2 - 111) * b ** 2 + (a ** 2 + 1) ** 2 * (15 * a ** 4 + 32 * a ** 2 - 23) * b)
+    num_part2 = (a ** 2 + 1) * inner / delta ** 2
+    coeff = (3 * (a ** 2 + 1) ** 4 - 2 * a * (3 * a ** 4 + 10 * a ** 2 + 15) * (a ** 2 + 1) * b
+             + 6 * (a ** 6 + 5 * a ** 4 + 15 * a ** 2 - 5) * b ** 2)
+    num_part3 = coeff * (m.atan(b / (a ** 2 - a * b + 1)) + m.atan(1 / a))
+    return -(num_part1 + num_part2 + num_part3) / (8 * (a ** 2 + 1) ** 4 * b)
+
+
+def comp_series(a, b):
+    """Large-b form combine() falls back to (atomic_compton.py:83-86)."""
+    return (-145 + 1015 * a ** 2 + 330 * a * b + 64 * b ** 2) / (420 * b ** 8)
+
+
+def _expr2(a, b, m=_NumpyMath):
+    """atomic_compton.py:46-79."""
+    delta = (a - b) ** 2 + 1
+    denominator = 8 * (a ** 2 + 1) ** 4 * b * delta ** 2
+    term1_inner = (a * (a ** 2 + 1) ** 3 * (3 * a ** 2 + 5) - 6 * (a ** 4 + 4 * a ** 2 - 5) * b ** 5
+                   + 4 * a * (6 * a ** 4 + 23 * a ** 2 - 31) * b ** 4 - (a - 1) * (a + 1) * (39 * a ** 4 + 190 * a ** 2 + 55) * b ** 3
+                   + a * (a ** 2 + 1) * (33 * a ** 4 + 98 * a ** 2 - 111) * b ** 2 - (a ** 2 + 1) ** 2 * (15 * a ** 4 + 32 * a ** 2 - 23) * b)
+    term1 = (a ** 2 + 1) * term1_inner
+    coeff = (3 * (a ** 2 + 1) ** 4 - 2 * a * (3 * a ** 4 + 10 * a ** 2 + 15) * (a ** 2 + 1) * b
+             + 6 * (a ** 6 + 5 * a ** 4 + 15 * a ** 2 - 5) * b ** 2)
+    term2 = m.pi * coeff * delta ** 2
+    term3 = coeff * delta ** 2 * m.atan(1 / a)
+    log_term = m.log(delta) - 2 * m.log(b)
+    term4 = delta ** 2 * (8 * b * (a ** 2 - 6 * a * b + 1) * log_term + coeff * m.atan(b / (a ** 2 - a * b + 1)))
+    return (term1 + term2 - term3 - term4) / denominator
+
+
+def sigma_atomic_comp(k, mV, Zeff):
+    """atomic_compton.py:187-214."""
+    m_e = m_electron
+    Lambda = Zeff * alpha_em * m_e
+    a = m_e / Lambda
+    b = mV ** 2 / (2 * k * Lambda)
+    s = 2 * k * m_e + m_e ** 2
+    beta = 2 * alpha_em / np.pi * (np.log(s / m_e ** 2) - 1.0)
+    if beta < 0.0:
+        beta = 0.0
+    if a ** 2 + 1 < a * b:
+        c = _expr1(a, b)
+        if c < 0.0 or (a ** 2 + 1 < 0.01 * a * b):
+            c = comp_series(a, b)
+    else:
+        c = _expr2(a, b)
+    return (4.0 * np.pi * alpha_em) * ((mV ** 2 + 2 * m_e ** 2) * (2 / 3 / Lambda / m_e) * (1 / k ** 2) * (beta / 4) * c)
+
+
+class LogInterp:
+    """interpolate1d(xspace='log', yspace='log', fill_value=-20) (dark_shower.py:31-46)."""
+
+    def __init__(self, x, y):
+        self.lx = np.log10(np.asarray(x) + 1e-20)
+        self.ly = np.log10(np.asarray(y) + 1e-20)
+
+    def __call__(self, x):
+        l = np.log10(x)
+        v = np.interp(l, self.lx, self.ly)
+        return 10 ** np.where((l < self.lx[0]) | (l > self.lx[-1]), -20.0, v)
+
+
+class OracleDark:
+    def __init__(self, sm_tables, mV_in_GeV, mode="exact", Zeff=29.508, kinetic_mixing=1.0, table_set="100GeV",
+                 below="clamp", use_cache=True):
+        z, meta = load_pack("dark", table_set)
+        self.T = sm_tables
+        T = sm_tables
+        self.mV_list = meta["masses"]
+        if mode == "exact":
+            self.mV = closest_lesser_value(self.mV_list, mV_in_GeV, below)
+            self.mV_est = self.mV
+        else:
+            self.mV = mV_in_GeV
+            self.mV_est = closest_lesser_value(self.mV_list, mV_in_GeV, below)
+        self.Zeff = Zeff
+        self.eps2 = kinetic_mixing ** 2
+        key, mat = repr(self.mV_est), T.material
+
+        def xs(P):
+            return np.array(z[f"xsec__{P}__{key}__{mat}"])
+
+        DBS = xs("DarkBrem")
+        while DBS[0, 1] == 0.0:
+            DBS = DBS[1:]
+        DMB = xs("DarkMuonBrem")
+        while DMB[0, 1] == 0.0:
+            DMB = DMB[1:]
+        self.DBS, self.DMB = DBS, DMB
+        self.E_res = (self.mV ** 2 - 2 * m_electron ** 2) / (2 * m_electron)
+        self.E_C = self.mV ** 2 / (2 * m_electron) + self.mV
+        self.min_dark = {11: {"DarkBrem": DBS[0, 0]}, -11: {"DarkBrem": DBS[0, 0], "DarkAnn": self.E_res / 1000.0},
+                         22: {"DarkComp": self.E_C / 1000.0}, 13: {"DarkMuonBrem": DMB[0, 0]},
+                         -13: {"DarkMuonBrem": DMB[0, 0]}}
+        Emin = max(T.min_calc[-11], 0.001 * self.E_res)
+        Ea = np.logspace(np.log10(Emin), np.log10(DBS[-1, 0]), 200)
+        self.DAS = np.column_stack([Ea, [sigma_atomic(e, self.mV, Zeff) for e in Ea]])
+        Emin = max(T.min_calc[22], 0.001 * self.E_C)
+        Ec = np.logspace(np.log10(Emin), np.log10(xs("DarkComp")[-1, 0]), 200)
+        self.DCS = np.column_stack([Ec, [sigma_atomic_comp(e, self.mV, Zeff) for e in Ec]])
+        nZ, ne = T.nZ, T.ne
+        self.NS = {"DarkBrem": LogInterp(DBS[:, 0], nZ * GeVsqcm2 * DBS[:, 1]),
+                   "DarkAnn": LogInterp(Ea, ne * GeVsqcm2 * self.DAS[:, 1]),
+                   "DarkComp": LogInterp(Ec, ne * GeVsqcm2 * self.DCS[:, 1]),
+                   "DarkMuonBrem": LogInterp(DMB[:, 0], nZ * GeVsqcm2 * DMB[:, 1])}
+        # interaction integrals via quad of the interpolants (shower.py:298-320)
+        self.II = {}
+        for P in ("Brem", "Ann", "Moller", "Bhabha", "MuonE"):
+            xg = T.nsigma_tab[P][0]
+            f = lambda e, P=P: float(T.nsigma(P, e))
+            vals = np.array([quad(f, xg[0], xg[i], full_output=1, limit=200, points=xg[:i + 1][1:-1] if i > 1 else None)[0]
+                             if i > 0 else 0.0 for i in range(len(xg))])
+            self.II[P] = (xg, vals)
+        xg = T.nsigma_tab["MuonBrem"][0]
+        f = lambda e: float(T.nsigma("MuonE", e))
+        mx = T.nsigma_tab["MuonE"][0]
+        self.II["MuonBrem"] = (xg, np.array([quad(f, xg[0], xg[i], full_output=1, limit=200,
+                                                  points=mx[(mx > xg[0]) & (mx < xg[i])])[0] if i > 0 else 0.0
+                                             for i in range(len(xg))]))
+        self.dEdxT = T.dEdx
+        self.z, self.key = z, key
+        self.use_cache = use_cache
+        self._grids = {}
+        for P in DARK_CODE:
+            gk = f"{P}__{key}"
+            if f"E__{gk}" in z.files:
+                self._grids[P] = Grid(z[f"E__{gk}"], z[f"nodes__{gk}"], z[f"nodeoff__{gk}"], z[f"nodecnt__{gk}"])
+
+    def _ii(self, P, E):
+        xg, yg = self.II[P]
+        return interp0(xg, yg, E)
+
+    def exp_factor(self, pid, E, Ei):
+        chans = {11: ("Brem", "Moller"), -11: ("Brem", "Ann", "Bhabha"), 13: ("MuonBrem", "MuonE")}[pid if pid != -13 else 13]
+        d = sum(float(self._ii(P, Ei) - self._ii(P, E)) for P in chans)
+        if d < 0.0 or E > Ei:
+            return 0.0
+        return math.exp(-d / self.dEdxT / cmtom)
+
+    def integrand(self, proc, pid, E, Ei):
+        k = self.T.dEdx * cmtom
+        n = float(self.NS[proc](E))
+        if proc != "DarkAnn" and n < 1.0e-18:
+            return 0.0
+        return n / k * self.exp_factor(pid, E, Ei)
+
+    def weight(self, proc, pid, Ei, E0):
+        mfp = self.T.mfp(pid if pid != -13 else 13, Ei)
+        eb = max(Ei - 10 * mfp * self.dEdxT, E0)
+        f = lambda e: self.integrand(proc, pid, e, Ei)
+        return (quad(f, E0, eb, full_output=1)[0] + quad(f, eb, Ei, full_output=1)[0])
+
+    def drate(self, proc, pid, Ei, lo_table):
+        mfp = self.T.mfp(pid if pid != -13 else 13, Ei)
+        e = np.linspace(max(Ei - 10 * mfp * self.dEdxT, lo_table), Ei, 11)
+        f = lambda x: self.integrand(proc, pid, x, Ei)
+        w = np.array([quad(f, e[i], e[i + 1], full_output=1)[0] for i in range(10)])
+        return np.column_stack([0.5 * (e[1:] + e[:-1]), w])

@@ -1,0 +1,391 @@
+"""Host-side dark-sector tables (DarkShower initialisation).
+
+Builds, once per (table set, material, m_V), what DarkShower.__init__ builds in
+the reference (src/PETITE/dark_shower.py:85-593):
+
+* mass selection (closest_lesser_value, 158-176),
+* dark cross-section tables and thresholds (219-245),
+* bound-electron annihilation and Compton cross sections on their 200-point
+  grids (254-292, src/PETITE/atomic_annihilation.py, atomic_compton.py),
+* log-log n*sigma interpolants (294-309, interpolate1d 31-46),
+* track-length weights W(E_i) and the 10-bin interaction-energy rates
+  (drate) (311-593), from the survival factor of the SM interaction
+  integrals (src/PETITE/shower.py:298-354).
+
+The weights and rates are integrals the reference evaluates with
+scipy.integrate.quad.  Here they use Gauss-Legendre panels that follow the
+exponential survival factor (width ~ dE/dx x mean free path) and break at
+every table node; caches shipped with the reference (dark_weights.pkl,
+dark_drate.pkl) are used when they cover the (mass, material) pair, as the
+reference does.
+"""
+import numpy as np
+
+from .constants import GeVsqcm2, alpha_em, cmtom, m_electron, m_muon
+from .tables import MaterialTables, _load
+
+DARK_PROCESSES = ["DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"]
+_GL_X, _GL_W = np.polynomial.legendre.leggauss(8)
+
+
+def closest_lesser_value(values, x, below="clamp"):
+    """dark_shower.py:158-165.  For x below the lightest mass the reference
+    indexes [-1] and wraps to the heaviest (below='wrap'); below='clamp'
+    takes the lightest instead."""
+    arr = np.asarray(values, dtype=np.float64)
+    i = int(np.argmin(np.abs(arr - x)))
+    if arr[i] <= x:
+        return float(arr[i])
+    if i == 0 and below == "clamp":
+        return float(arr[0])
+    return float(arr[i - 1])
+
+
+# ---------------------------------------------------------------- bound e- --
+def _atomic_ann(k, mV, Zeff):
+    """sigma of e+ e-(bound) -> V, tree + radiative tail (atomic_annihilation.py:70-104).
+    That module uses its own alpha = 1/137 and m_e = 0.511 MeV.  The closed form
+    cancels strongly near threshold, so each term is formed in the reference's
+    order (scalar float64) to reproduce its values there."""
+    return np.array([_atomic_ann1(np.float64(kk), mV, Zeff) for kk in np.atleast_1d(k)])
+
+
+def _atomic_ann1(k, mV, Zeff):
+    al, me = 1 / 137, 0.511e-3
+    lam = Zeff * al * me
+    a = me / lam
+    b = mV ** 2 / (2 * k * lam)
+    pre = (4.0 * np.pi * al) * ((mV ** 2 + 2 * me ** 2) * (2 / 3 / lam / me) * (1 / k ** 2) * (1 / ((a - b) ** 2 + 1) ** 3))
+    s = 2 * k * me + me ** 2
+    beta = 2 * al / np.pi * (np.log(s / me ** 2) - 1.0)
+    tail = (4.0 * np.pi * al) * ((mV ** 2 + 2 * me ** 2) * (2 / 3 / lam / me) * (1 / k ** 2) * (beta / 2) * _fancy1(a, b))
+    return pre + tail
+
+
+def _fancy1(a, b):
+    # closed-form radiative-tail integral of atomic_annihilation.py:10-68 (series for b >= 1000)
+    if b >= 1000:
+        return (3081 / 280 - (3081 * a ** 2) / 40) / b ** 8 - (709 * a) / (35 * b ** 7) - 91 / (30 * b ** 6)
+    p1 = (1 + a ** 2)
+    p3 = (1 + a ** 2) ** 3
+    c5 = 5 + 3 * a ** 2
+    p2 = (1 + a ** 2) ** 2
+    e1 = -1 + 15 * a ** 2 + 12 * a ** 4
+    t1 = p1 * (a * p3 * c5 - p2 * e1 * b + a * (1 + a ** 2) * (-23 + 38 * a ** 2 + 21 * a ** 4) * b ** 2
+               + 4 * (5 + a ** 2 - 18 * a ** 4 - 6 * a ** 6) * b ** 3 + 3 * a * (3 + a ** 2) * (-1 + 7 * a ** 2) * b ** 4
+               + (7 - 29 * a ** 2 - 12 * a ** 4) * b ** 5 + a * (7 + 3 * a ** 2) * b ** 6)
+    poly = (3 * (1 + a ** 2) ** 6 - a * (1 + a ** 2) ** 3 * (3 + 26 * a ** 2 + 15 * a ** 4) * b
+            + 3 * (1 + a ** 2) ** 2 * (-7 + 21 * a ** 2 + 23 * a ** 4 + 11 * a ** 6) * b ** 2
+            - 3 * a * (1 + a ** 2) * (7 + 69 * a ** 2 + 45 * a ** 4 + 15 * a ** 6) * b ** 3
+            + (-11 + 132 * a ** 2 + 9 * a ** 4 * (38 + 5 * a ** 2 * (4 + a ** 2))) * b ** 4
+            - 3 * a * (11 + 73 * a ** 2 + 41 * a ** 4 + 11 * a ** 6) * b ** 5
+            + 3 * (-1 + 27 * a ** 2 + 17 * a ** 4 + 5 * a ** 6) * b ** 6 - a * (15 + 10 * a ** 2 + 3 * a ** 4) * b ** 7)
+    t4 = (1 + a ** 2) ** 3 + 6 * a * (1 + a ** 2) ** 2 * b - 3 * (1 + 6 * a ** 2 + 5 * a ** 4) * b ** 2 + \
+        4 * a * (3 + 5 * a ** 2) * b ** 3 - 3 * (1 + 5 * a ** 2) * b ** 4 + 6 * a * b ** 5 - b ** 6
+    num = (t1 + poly * np.arctan2(a, 1) + poly * np.arctan2(1 + a ** 2 - a * b, b)
+           + 4 * b * (t4 * (np.log(1 + (a - b) ** 2) - 2 * np.log(b))))
+    return num / (8 * (1 + a ** 2) ** 3 * (1 + (a - b) ** 2) ** 3 * b)
+
+
+def _atomic_comp(k, mV, Zeff):
+    """sigma of gamma e-(bound) -> e- V (atomic_compton.py:94-109 with combine/expr1/expr2),
+    term order as in the reference (see _atomic_ann)."""
+    return np.array([_atomic_comp1(np.float64(kk), mV, Zeff) for kk in np.atleast_1d(k)])
+
+
+def _atomic_comp1(k, mV, Zeff):
+    me, al = m_electron, alpha_em
+    lam = Zeff * al * me
+    a = me / lam
+    b = mV ** 2 / (2 * k * lam)
+    s = 2 * k * me + me ** 2
+    beta = 2 * al / np.pi * (np.log(s / me ** 2) - 1.0)
+    if beta < 0.0:
+        beta = 0.0
+    return (4.0 * np.pi * al) * ((mV ** 2 + 2 * me ** 2) * (2 / 3 / lam / me) * (1 / k ** 2) * (beta / 4) * _combine1(a, b))
+
+
+def _inner(a, b):
+    return (-a * (a ** 2 + 1) ** 3 * (3 * a ** 2 + 5) + 6 * (a ** 4 + 4 * a ** 2 - 5) * b ** 5
+            - 4 * a * (6 * a ** 4 + 23 * a ** 2 - 31) * b ** 4 + (a - 1) * (a + 1) * (39 * a ** 4 + 190 * a ** 2 + 55) * b ** 3
+            - a * (a ** 2 + 1) * (33 * a ** 4 + 98 * a ** 2 - 111) * b ** 2 + (a ** 2 + 1) ** 2 * (15 * a ** 4 + 32 * a ** 2 - 23) * b)
+
+
+def _coeff(a, b):
+    return (3 * (a ** 2 + 1) ** 4 - 2 * a * (3 * a ** 4 + 10 * a ** 2 + 15) * (a ** 2 + 1) * b
+            + 6 * (a ** 6 + 5 * a ** 4 + 15 * a ** 2 - 5) * b ** 2)
+
+
+def _combine1(a, b):
+    d = (a - b) ** 2 + 1
+    lt = np.log(d) - 2 * np.log(b)
+    if a ** 2 + 1 < a * b:
+        # region above the kinematic knee: expr1, with its large-b series
+        num = (8 * b * (a ** 2 - 6 * a * b + 1) * lt + (a ** 2 + 1) * _inner(a, b) / d ** 2
+               + _coeff(a, b) * (np.arctan(b / (a ** 2 - a * b + 1)) + np.arctan(1 / a)))
+        e1 = -num / (8 * (a ** 2 + 1) ** 4 * b)
+        if e1 < 0.0 or (a ** 2 + 1 < 0.01 * a * b):
+            e1 = (-145 + 1015 * a ** 2 + 330 * a * b + 64 * b ** 2) / (420 * b ** 8)
+        return e1
+    c = _coeff(a, b)
+    t1 = (a ** 2 + 1) * -_inner(a, b)
+    t2 = np.pi * c * d ** 2
+    t3 = c * d ** 2 * np.arctan(1 / a)
+    t4 = d ** 2 * (8 * b * (a ** 2 - 6 * a * b + 1) * lt + c * np.arctan(b / (a ** 2 - a * b + 1)))
+    return (t1 + t2 - t3 - t4) / (8 * (a ** 2 + 1) ** 4 * b * d ** 2)
+
+
+# ------------------------------------------------------------ interpolants --
+class LogLog:
+    """interpolate1d(xspace='log', yspace='log', fill_value=-20) (dark_shower.py:31-46)."""
+
+    def __init__(self, x, y):
+        self.lx = np.log10(np.asarray(x, dtype=np.float64) + 1e-20)
+        self.ly = np.log10(np.asarray(y, dtype=np.float64) + 1e-20)
+
+    def __call__(self, E):
+        l = np.log10(np.asarray(E, dtype=np.float64))
+        out = np.interp(l, self.lx, self.ly)
+        out = np.where((l < self.lx[0]) | (l > self.lx[-1]), -20.0, out)
+        return 10.0 ** out
+
+
+def _lin0(xg, yg, x):
+    x = np.asarray(x, dtype=np.float64)
+    return np.where((x < xg[0]) | (x > xg[-1]), 0.0, np.interp(x, xg, yg))
+
+
+def _cumint_pl(xg, yg, nodes):
+    """Exact integral from nodes[0] to each node of the piecewise-linear
+    interpolant (xg, yg), zero outside [xg[0], xg[-1]]."""
+    nodes = np.asarray(nodes, dtype=np.float64)
+    pts = np.union1d(nodes, xg[(xg > nodes[0]) & (xg < nodes[-1])])
+    f = _lin0(xg, yg, pts)
+    seg = 0.5 * (f[1:] + f[:-1]) * np.diff(pts)
+    # a jump of the fill at the ends of xg would need the one-sided limits
+    cum = np.concatenate([[0.0], np.cumsum(seg)])
+    return np.interp(nodes, pts, cum)
+
+
+class DarkTables:
+    """Dark tables of one material, table set and mass."""
+
+    def __init__(self, sm: MaterialTables, mV, mode="exact", Zeff=29.508, bound_electron=True, kinetic_mixing=1.0,
+                 below="clamp"):
+        if not bound_electron:
+            raise NotImplementedError("only bound_electron=True (the reference's default) is implemented")
+        z, meta = _load("dark", sm.table_set)
+        self.meta = meta
+        self.sm = sm
+        self.material = sm.material
+        self.mV_list = list(meta["masses"])
+        if mode == "exact":
+            self.mV = closest_lesser_value(self.mV_list, mV, below)
+            self.mV_est = self.mV
+        elif mode == "approx":
+            self.mV = float(mV)
+            self.mV_est = closest_lesser_value(self.mV_list, mV, below)
+        else:
+            raise Exception("Mode not valid. Chose exact or approx.")
+        self.Zeff = Zeff
+        self.kinetic_mixing = kinetic_mixing
+        self.eps2 = kinetic_mixing ** 2  # g_e^2 / (4 pi alpha) (dark_shower.py:109-111)
+        key = repr(self.mV_est)
+        mat = self.material
+
+        def xs(P):
+            return np.array(z[f"xsec__{P}__{key}__{mat}"], dtype=np.float64)
+
+        dbs = xs("DarkBrem")
+        while dbs[0, 1] == 0.0:
+            dbs = dbs[1:]
+        dmb = xs("DarkMuonBrem")
+        while dmb[0, 1] == 0.0:
+            dmb = dmb[1:]
+        self.xsec = {"DarkBrem": dbs, "DarkAnn_free": xs("DarkAnn"), "DarkComp_free": xs("DarkComp"), "DarkMuonBrem": dmb}
+        mV_ = self.mV
+        self.E_res = (mV_ ** 2 - 2 * m_electron ** 2) / (2 * m_electron)
+        self.E_C = mV_ ** 2 / (2 * m_electron) + mV_
+        self.min_calc_dark = {11: {"DarkBrem": dbs[0, 0]},
+                              -11: {"DarkBrem": dbs[0, 0], "DarkAnn": self.E_res / 1000.0},
+                              22: {"DarkComp": self.E_C / 1000.0},
+                              13: {"DarkMuonBrem": dmb[0, 0]}, -13: {"DarkMuonBrem": dmb[0, 0]}}
+        # bound-electron grids (dark_shower.py:254-292)
+        Ea = np.logspace(np.log10(max(sm.min_calc[-11], 0.001 * self.E_res)), np.log10(dbs[-1, 0]), 200)
+        self.xsec["DarkAnn"] = np.column_stack([Ea, _atomic_ann(Ea, mV_, Zeff)])
+        Ec = np.logspace(np.log10(max(sm.min_calc[22], 0.001 * self.E_C)), np.log10(xs("DarkComp")[-1, 0]), 200)
+        self.xsec["DarkComp"] = np.column_stack([Ec, _atomic_comp(Ec, mV_, Zeff)])
+        nZ, ne = sm.nZ, sm.ne
+        self.ns = {"DarkBrem": LogLog(dbs[:, 0], nZ * GeVsqcm2 * dbs[:, 1]),
+                   "DarkAnn": LogLog(Ea, ne * GeVsqcm2 * self.xsec["DarkAnn"][:, 1]),
+                   "DarkComp": LogLog(Ec, ne * GeVsqcm2 * self.xsec["DarkComp"][:, 1]),
+                   "DarkMuonBrem": LogLog(dmb[:, 0], nZ * GeVsqcm2 * dmb[:, 1])}
+        self._build_survival()
+        self._build_weights(z, key)
+        self.grids = {}
+        for P in DARK_PROCESSES:
+            gk = f"{P}__{key}"
+            if f"E__{gk}" in z.files:
+                self.grids[P] = (np.array(z[f"E__{gk}"]), np.array(z[f"nodes__{gk}"], dtype=np.float32),
+                                 np.array(z[f"nodecnt__{gk}"]))
+        # n*sigma of the dark processes for the device (log-log -> tabulate as given)
+        self.nsigma_tab = {}
+
+    # -- survival factors (shower.py:298-354) ---------------------------------
+    def _build_survival(self):
+        sm = self.sm
+        ii = {}
+        for P in ("Brem", "PairProd", "Ann", "Comp", "Moller", "Bhabha", "MuonE"):
+            xg, yg = sm.nsigma_tab[P]
+            ii[P] = (xg, _cumint_pl(xg, yg, xg))
+        # the reference integrates n sigma(MuonE) on the MuonBrem grid (shower.py:316)
+        mbx = sm.nsigma_tab["MuonBrem"][0]
+        mex, mey = sm.nsigma_tab["MuonE"]
+        ii["MuonBrem"] = (mbx, _cumint_pl(mex, mey, mbx))
+        self.ii = ii
+        self.dEdxT = sm.dEdx  # GeV/m
+
+    def _II(self, P, E):
+        xg, yg = self.ii[P]
+        return _lin0(xg, yg, E)
+
+    def survival(self, pid, E, Ei):
+        """_electron/_positron/_muon_exponential_factor (shower.py:322-354), vectorised in E."""
+        chans = {11: ("Brem", "Moller"), -11: ("Brem", "Ann", "Bhabha"), 13: ("MuonBrem", "MuonE"),
+                 -13: ("MuonBrem", "MuonE")}[pid]
+        E = np.asarray(E, dtype=np.float64)
+        d = sum(self._II(P, Ei) - self._II(P, E) for P in chans)
+        out = np.exp(-d / self.dEdxT / cmtom)
+        return np.where((d < 0.0) | (E > Ei), 0.0, out)
+
+    def mfp(self, pid, E):
+        sm = self.sm
+        chans = {11: ("Brem", "Moller"), -11: ("Brem", "Ann", "Bhabha"), 13: ("MuonE", "MuonBrem"),
+                 -13: ("MuonE", "MuonBrem")}[pid]
+        ns = float(sum(sm.nsigma(P, E) for P in chans))
+        return 1.0e12 if ns <= 0.0 else cmtom / ns
+
+    # -- weights and rates (dark_shower.py:311-593) -----------------------------
+    def _integrand(self, proc, pid, E, Ei):
+        k = self.sm.dEdx * cmtom  # GeV/cm
+        n = self.ns[proc](E)
+        if proc != "DarkAnn":
+            n = np.where(n < 1.0e-18, 0.0, n)
+        return n / k * self.survival(pid, E, Ei)
+
+    def _panels(self, lo, hi, Ei, scale, nodes):
+        """Panel edges on [lo, hi]: geometric in the distance below Ei, starting
+        at scale/8, plus every table node inside."""
+        if hi <= lo:
+            return np.array([lo, hi])
+        edges = [hi]
+        d = scale / 8.0
+        while Ei - d > lo:
+            if Ei - d < hi:
+                edges.append(Ei - d)
+            d *= 1.35
+        edges.append(lo)
+        inner = nodes[(nodes > lo) & (nodes < hi)]
+        return np.unique(np.concatenate([np.array(edges), inner]))
+
+    def integrate(self, proc, pid, lo, hi, Ei, nodes):
+        if hi <= lo:
+            return 0.0
+        scale = max(10.0 * self.mfp(pid, Ei) * self.dEdxT, 1e-9)
+        e = self._panels(lo, hi, Ei, scale / 10.0, nodes)
+        a, b = e[:-1], e[1:]
+        h = 0.5 * (b - a)
+        c = 0.5 * (b + a)
+        x = c[:, None] + h[:, None] * _GL_X[None, :]
+        f = self._integrand(proc, pid, x.ravel(), Ei).reshape(x.shape)
+        return float(np.sum(h * (f @ _GL_W)))
+
+    def _nodes(self, proc, pid):
+        sm = self.sm
+        chans = {11: ("Brem", "Moller"), -11: ("Brem", "Ann", "Bhabha"), 13: ("MuonBrem", "MuonE"),
+                 -13: ("MuonBrem", "MuonE")}[pid]
+        pts = [self.ii[P][0] for P in chans]
+        lx = self.ns[proc].lx
+        pts.append(10.0 ** lx)
+        return np.unique(np.concatenate(pts))
+
+    def weight_array(self, proc, pid):
+        """Track-length weights on the process's own energy grid (construct_*_weight_array)."""
+        if proc == "DarkAnn":
+            Ei_all = self.xsec["DarkAnn"][:, 0]
+        elif proc == "DarkMuonBrem":
+            Ei_all = self.xsec["DarkMuonBrem"][:, 0]
+        else:
+            Ei_all = self.xsec["DarkBrem"][:, 0]
+        lo = Ei_all[0]
+        nodes = self._nodes(proc, pid)
+        W = np.array([self.integrate(proc, pid, lo, Ei, Ei, nodes) for Ei in Ei_all])
+        return np.column_stack([Ei_all, W])
+
+    def drate(self, proc, pid, Ei, lo_table):
+        """_d_rate_d_E_*: 10 equal bins over [max(Ei - 10 mfp dE/dx, E_table0), Ei]."""
+        lo = max(Ei - 10.0 * self.mfp(pid, Ei) * self.dEdxT, lo_table)
+        edges = np.linspace(lo, Ei, 11)
+        nodes = self._nodes(proc, pid)
+        w = np.array([self.integrate(proc, pid, edges[i], edges[i + 1], Ei, nodes) for i in range(10)])
+        return np.column_stack([0.5 * (edges[1:] + edges[:-1]), w])
+
+    def _build_weights(self, z, key):
+        mat = self.material
+        cached = f"weights__brem_elec_weights__{key}__{mat}" in z.files
+        self.from_cache = cached
+        W = {}
+        if cached:
+            W["brem_elec"] = np.array(z[f"weights__brem_elec_weights__{key}__{mat}"])
+            W["brem_positron"] = np.array(z[f"weights__brem_positron_weights__{key}__{mat}"])
+            W["muon_brem"] = np.array(z[f"weights__muon_brem_weights__{key}__{mat}"])
+        else:
+            W["brem_elec"] = self.weight_array("DarkBrem", 11)
+            W["brem_positron"] = self.weight_array("DarkBrem", -11)
+            W["muon_brem"] = self.weight_array("DarkMuonBrem", 13)
+        W["annihilation"] = self.weight_array("DarkAnn", -11)  # bound: always recomputed (dark_shower.py:450-451)
+        self.W = W
+        D = {}
+        dbE = self.xsec["DarkBrem"][:, 0]
+        dmE = self.xsec["DarkMuonBrem"][:, 0]
+        daE = self.xsec["DarkAnn"][:, 0]
+        for name, proc, pid, Es, lo in (("brem_elec", "DarkBrem", 11, dbE, dbE[0]),
+                                         ("brem_positron", "DarkBrem", -11, dbE, dbE[0]),
+                                         ("muon_brem", "DarkMuonBrem", 13, dmE, dmE[0])):
+            ck = f"drate__{name}_drate__{key}__{mat}"
+            if ck in z.files:
+                arr = np.array(z[ck])
+                cnt = np.array(z[f"dratecnt__{name}_drate__{key}__{mat}"])
+                D[name] = (np.array(z[f"drateE__{name}_drate__{key}__{mat}"]), arr[:, :10, :], cnt)
+            else:
+                D[name] = (Es, np.stack([self.drate(proc, pid, Ei, lo) for Ei in Es]), np.full(len(Es), 10))
+        D["annihilation"] = (daE, np.stack([self.drate("DarkAnn", -11, Ei, daE[0]) for Ei in daE]),
+                             np.full(len(daE), 10))
+        self.D = D
+
+    def bsm_weight(self, pid, E0, process):
+        """GetBSMWeights (dark_shower.py:595-647) for arrays of (pid, E0)."""
+        pid = np.asarray(pid)
+        E0 = np.asarray(E0, dtype=np.float64)
+        out = np.zeros(np.broadcast(pid, E0).shape)
+        sm = self.sm
+        if process == "DarkComp":
+            m = (pid == 22) & (E0 >= self.min_calc_dark[22]["DarkComp"]) & (E0 >= sm.min_calc[22])
+            den = sm.nsigma("PairProd", E0) + sm.nsigma("Comp", E0)
+            with np.errstate(all="ignore"):
+                v = self.ns["DarkComp"](E0) / den
+            out = np.where(m, v, out)
+        elif process == "DarkBrem":
+            for p, name in ((11, "brem_elec"), (-11, "brem_positron")):
+                m = (pid == p) & (E0 >= self.min_calc_dark[p]["DarkBrem"])
+                W = self.W[name]
+                out = np.where(m, _lin0(W[:, 0], W[:, 1], E0), out)
+        elif process == "DarkAnn":
+            m = (pid == -11) & (E0 >= self.min_calc_dark[-11]["DarkAnn"])
+            W = self.W["annihilation"]
+            out = np.where(m, _lin0(W[:, 0], W[:, 1], E0), out)
+        elif process == "DarkMuonBrem":
+            m = (np.abs(pid) == 13) & (E0 >= self.min_calc_dark[13]["DarkMuonBrem"])
+            W = self.W["muon_brem"]
+            out = np.where(m, _lin0(W[:, 0], W[:, 1], E0), out)
+        return self.eps2 * out
