@@ -162,6 +162,70 @@ int petite_eval_points(PetiteCtx* ctx, const int32_t* proc, const double* E, con
 int petite_run_showers(PetiteCtx* ctx, const PetitePrimary* prim, int64_t n, uint64_t seed, PetiteStore* store,
                        double* shower_energy, PetiteStats* stats, PetiteStream stream);
 
+/* ----------------------------------------------------------- dark sector --
+ * Weight families (GetBSMWeights, src/PETITE/dark_shower.py:595-647):
+ * 0 e- DarkBrem, 1 e+ DarkBrem, 2 mu DarkMuonBrem, 3 e+ DarkAnn (bound electron). */
+#define PETITE_DARK_FAMILIES 4
+#define PETITE_DARK_DRATE_BINS 10
+
+/* Dark tables of one (material, m_V) pair, built by DarkShower.__init__
+ * (src/PETITE/dark_shower.py:85-593).  Offsets index the float64 blob given to
+ * petite_dark_setup. */
+typedef struct {
+  int32_t w_n[PETITE_DARK_FAMILIES];   /* weight arrays: E[n] at w_off, W[n] at w_off + n (linear, 0 outside) */
+  int64_t w_off[PETITE_DARK_FAMILIES];
+  int32_t d_n[PETITE_DARK_FAMILIES];   /* drate tables (_d_rate_d_E_*, 311-593): saved energies E[n] at d_off, */
+  int64_t d_off[PETITE_DARK_FAMILIES]; /* then bin counts cnt[n], then rows [n][10][2] (bin energy, rate) */
+  int32_t comp_n;                      /* DarkComp n*sigma [1/cm]: log10 E[n] at comp_off, log10 value[n] after */
+  int32_t n_active;                    /* active processes, in the caller's order (dark process codes 8..11) */
+  int64_t comp_off;
+  int32_t active[8];
+  double eps2;                         /* g_e^2 / (4 pi alpha) = kinetic_mixing^2 */
+  double E_res, E_C;                   /* bound-electron thresholds (dark_shower.py:234-235) */
+  double min_brem, min_ann, min_comp, min_muon_brem; /* _minimum_calculable_dark_energy (243-245) */
+} PetiteDarkDesc;
+
+/* Dark vectors written by petite_dark_reweight, one entry per (SM particle,
+ * active process) with a positive weight: p (E, px, py, pz), r (x, y, z,
+ * weight), parent (SM store slot) and meta (process code | 0x100 when the
+ * entry holds a vector; entries whose production returned None in the
+ * reference keep 0 there). */
+typedef struct {
+  float* p;
+  float* r;
+  int32_t* parent;
+  int32_t* meta;
+  int64_t capacity;
+} PetiteDarkOut;
+
+typedef struct {
+  int64_t n_entries;    /* entries written (SM particle, process pairs with weight > 0) */
+  int64_t n_vectors;    /* entries that hold a vector */
+  int64_t n_draws;      /* draw_dark_sample calls */
+  int64_t n_trials;     /* their candidates */
+  int64_t n_no_sample;  /* draws that hit max_trials ("No Sample Found", dark_shower.py:697-700) */
+  int64_t n_zero_rate;  /* drate rows that sum to zero (722-750) */
+  int64_t n_pe_fail;    /* bound-electron momentum draws that found no sample in 65536 pairs */
+  int64_t pad;
+  double yield[4];      /* sum of vector weights per dark process (DarkBrem, DarkAnn, DarkComp, DarkMuonBrem) */
+  double ms_select, ms_draw, ms_emit;
+  int64_t launches;
+} PetiteDarkStats;
+
+/* Attach the dark tables to a context whose table descriptor carries the dark
+ * grids, m_V and Zeff (blob: host float64 array of n doubles). */
+int petite_dark_setup(PetiteCtx* ctx, const PetiteDarkDesc* desc, const double* blob, int64_t n);
+
+/* Dark vectors of a shower record: replaces the loop of
+ * DarkShower.generate_dark_shower (src/PETITE/dark_shower.py:806-849) with
+ * GetBSMWeights (595-647), produce_bsm_particle (721-804), draw_dark_sample
+ * (649-704) and draw_pe_sample (706-719) for every (SM particle, active
+ * process) pair.  sm: the first n_sm records of a store filled by
+ * petite_run_showers (or by the caller).  PETITE_ERR_CAPACITY when out holds
+ * fewer entries than needed (stats->n_entries holds the number). */
+int petite_dark_reweight(PetiteCtx* ctx, const PetiteStore* sm, int64_t n_sm, uint64_t seed, PetiteDarkOut* out,
+                         PetiteDarkStats* stats, PetiteStream stream);
+
 #ifdef __cplusplus
 }
 #endif

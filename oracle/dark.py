@@ -265,3 +265,164 @@ class OracleDark:
         f = lambda x: self.integrand(proc, pid, x, Ei)
         w = np.array([quad(f, e[i], e[i + 1], full_output=1)[0] for i in range(10)])
         return np.column_stack([0.5 * (e[1:] + e[:-1]), w])
+
+
+# ------------------------------------------------------------------ sampling --
+# Random numbers of one (SM particle, dark process) pair come from its dark
+# stream dsid = derive_sid(seed, sid, code, PURPOSE_DARK):
+#   INTERACT block 0: x drate bin, y MCS radius, z MCS azimuth (sign: bit 0 of z), w kinematic azimuth
+#   INTERACT block 1: x bound-electron cos(theta_e) (2u - 1), y rotation azimuth of the bound-electron frame
+#   BOUND_E block k:  (x, y) and (z, w) accept/reject pairs of the bound-electron momentum
+#   VEGAS_A / VEGAS_B: the accept/reject candidates of draw_dark_sample
+W_FAMILY = {("DarkBrem", 11): "brem_elec", ("DarkBrem", -11): "brem_positron", ("DarkAnn", -11): "annihilation",
+            ("DarkMuonBrem", 13): "muon_brem", ("DarkMuonBrem", -13): "muon_brem"}
+PE_MAX_PAIRS = 1 << 16
+
+
+def electron_wave_function(pe, Zeff):
+    """dark_shower.py:706-708."""
+    lam = alpha_em * Zeff * m_electron
+    return 32 / np.pi * lam ** 5 * pe ** 2 / (pe ** 2 + lam ** 2) ** 4
+
+
+def draw_pe(seed, dsid, Zeff):
+    """draw_pe_sample (dark_shower.py:710-719): uniform pe in [0, 1 MeV), accepted
+    against the 1s momentum density normalised at its maximum."""
+    c = electron_wave_function(alpha_em * Zeff * m_electron / math.sqrt(3), Zeff)
+    for k in range(PE_MAX_PAIRS // 2):
+        u = philox.to_uniform(philox.words(seed, dsid, k, philox.PURPOSE_BOUND_E))
+        for x, a in ((u[0], u[1]), (u[2], u[3])):
+            pe = 1e-3 * x
+            if a < electron_wave_function(pe, Zeff) / c:
+                return pe
+    return None
+
+
+def drate_pick(Es, rows, cnt, E0, u):
+    """Interaction energy of produce_bsm_particle (dark_shower.py:741-750): the
+    saved energy Ei (largest <= E0, clamped to the table), a bin chosen with
+    probability rate/sum (numpy.random.choice: first cdf entry above u),
+    shifted by E0 - Ei.  None when the rates sum to zero."""
+    Es = np.asarray(Es)
+    if E0 < Es.min():
+        i = int(np.argmin(Es))
+    elif E0 > Es.max():
+        i = int(np.argmax(Es))
+    else:
+        i = int(np.nonzero(Es == np.max(Es[Es <= E0]))[0][0])
+    c = int(cnt[i])
+    e_c, rate = rows[i, :c, 0], rows[i, :c, 1]
+    tot = np.sum(rate)
+    if tot == 0.0:
+        return None
+    cdf = np.cumsum(rate / tot)
+    cdf = cdf / cdf[-1]
+    k = min(int(np.searchsorted(cdf, u, side="right")), c - 1)
+    return e_c[k] + (E0 - Es[i])
+
+
+class OracleDarkSampler:
+    """GetBSMWeights (dark_shower.py:595-647), produce_bsm_particle (721-804) and
+    the loop of generate_dark_shower (806-849) over a fixed SM record.
+
+    Inputs: an OracleDark (cross sections and thresholds), the weight arrays
+    W[name] = (n, 2) and drate tables D[name] = (energies, rows (n, 10, 2),
+    counts) that DarkShower builds at initialisation, and the max_F arrays of
+    the dark maps."""
+
+    def __init__(self, od, W, D, maxf, rescale_MCS=1.0, maxF_fudge=1.0, max_trials=int(1e4) * 300):
+        self.od, self.T = od, od.T
+        self.W, self.D, self.maxf = W, D, maxf
+        self.rescale, self.fudge, self.max_trials = rescale_MCS, maxF_fudge, max_trials
+
+    def weight(self, pid, E0, process):
+        od, T = self.od, self.T
+        if pid not in (11, -11, 13, -13, 22):
+            return 0.0
+        thr = od.min_dark[pid]
+        if process not in thr or E0 < thr[process]:
+            return 0.0
+        if pid == 22:
+            if process != "DarkComp" or E0 < T.min_calc[22]:
+                return 0.0
+            return od.eps2 * float(od.NS["DarkComp"](E0)) / float(T.nsigma("PairProd", E0) + T.nsigma("Comp", E0))
+        name = W_FAMILY.get((process, pid))
+        if name is None:
+            return 0.0
+        W = self.W[name]
+        return od.eps2 * float(interp0(W[:, 0], W[:, 1], E0))
+
+    def produce(self, rec, process, wg, seed):
+        """rec: dict with pid, p0, pf, rf, weight, sid.  Returns the V record or None."""
+        od, T = self.od, self.T
+        code = DARK_CODE[process]
+        pid = rec["pid"]
+        dsid = philox.derive_sid(seed, rec["sid"], code, philox.PURPOSE_DARK)
+        w0 = philox.words(seed, dsid, 0, philox.PURPOSE_INTERACT)
+        u0 = philox.to_uniform(w0)
+        p0 = np.asarray(rec["p0"], dtype=np.float64)
+        pf = np.asarray(rec["pf"], dtype=np.float64)
+        name = W_FAMILY.get((process, pid)) if process != "DarkComp" else None
+        if name is not None:
+            E0 = p0[0]
+            E_int = drate_pick(*self.D[name], E0, u0[0])
+            if E_int is None:
+                return None
+            dist = (E0 - E_int) / T.dEdx
+            sign = 1.0 if int(w0[2]) & 1 else -1.0
+            # beta from the on-shell momentum of the record's energy and mass
+            pn = float(np.linalg.norm(p0[1:]))
+            pon = math.sqrt(max(E0 * E0 - rec["mass"] ** 2, 0.0))
+            pin = np.concatenate([[E0], p0[1:] * (pon / pn)]) if pn > 0 else p0
+            ps = scatter(pin, T.rho * (dist / cmtom), T.A, T.Z, self.rescale, m_electron, u0[1], u0[2], sign)
+            pf = lose_energy(ps, rec["mass"], E0 - E_int)
+        E = pf[0]
+        RM = rotation_matrix(pf)
+        bound = process in ("DarkAnn", "DarkComp")
+        mV = od.mV
+        if process == "DarkAnn" and E <= od.E_res:
+            V = np.array([math.sqrt(E ** 2 - m_electron ** 2 + mV ** 2), 0.0, 0.0, math.sqrt(E ** 2 - m_electron ** 2)])
+        elif process == "DarkComp" and E <= od.E_C:
+            V = np.array([math.sqrt(E ** 2 + mV ** 2), 0.0, 0.0, E])
+        else:
+            grid = od._grids[process]
+            ie = grid.lu_key(E)
+            ev = T.event_info(process, E, mV)
+            ev["Zeff"] = od.Zeff
+            x, trials, _ = vegas.draw(grid, process, E, ev, self.maxf[process][ie] * self.fudge, seed, dsid,
+                                      max_trials=self.max_trials)
+            if x is None:
+                return None
+            if process == "DarkComp":
+                u1 = philox.to_uniform(philox.words(seed, dsid, 1, philox.PURPOSE_INTERACT))
+                pe = draw_pe(seed, dsid, od.Zeff)
+                if pe is None:
+                    return None
+                c0 = -1.0 + 2.0 * u1[0]
+                s = m_electron ** 2 + 2 * E * (math.sqrt(m_electron ** 2 + pe ** 2) - c0 * pe)
+                Ee = (s - mV ** 2 + m_electron ** 2) / (2 * math.sqrt(s))
+                if Ee < m_electron:
+                    V = np.array([mV, 0.0, 0.0, 0.0])
+                    wg = 0.0
+                else:
+                    V = compton_bound(E, x, mV, pe, c0, u0[3], u1[1])
+            elif process == "DarkAnn":
+                V = radiative_return(E, x, mV)
+            else:
+                V = l_to_lV(E, rec["mass"], x, u0[3], mV)[1]
+        p4 = np.concatenate([[V[0]], RM @ V[1:]])
+        return {"p": p4, "r": np.asarray(rec["rf"], dtype=np.float64)[:3], "weight": wg * rec["weight"],
+                "process": process + ("_bound" if bound else ""), "code": code}
+
+    def generate(self, records, processes, seed):
+        """generate_dark_shower's loop: one V per (SM particle, active process) with weight > 0."""
+        out = []
+        for i, rec in enumerate(records):
+            for P in processes:
+                wg = self.weight(rec["pid"], rec["p0"][0], P)
+                if wg > 0.0:
+                    v = self.produce(rec, P, wg, seed)
+                    if v is not None:
+                        v["parent"] = i
+                        out.append(v)
+        return out

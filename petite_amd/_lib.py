@@ -18,7 +18,10 @@ PETITE_ERR_NO_SAMPLE = 3
 PETITE_ERR_CAPACITY = 4
 
 EXPORTED = ["petite_ctx_create", "petite_ctx_destroy", "petite_last_error", "petite_maxf_scan", "petite_set_maxf",
-            "petite_vegas_draw", "petite_eval_points", "petite_run_showers", "petite_set_timing"]
+            "petite_vegas_draw", "petite_eval_points", "petite_run_showers", "petite_set_timing", "petite_dark_setup",
+            "petite_dark_reweight"]
+DARK_FAMILIES = ["brem_elec", "brem_positron", "muon_brem", "annihilation"]
+DARK_DRATE_BINS = 10
 
 
 class GridDesc(C.Structure):
@@ -62,6 +65,31 @@ class Stats(C.Structure):
         return {f: (float(getattr(self, f)) if t is C.c_double else int(getattr(self, f))) for f, t in self._fields_}
 
 
+class DarkDesc(C.Structure):
+    _fields_ = [("w_n", C.c_int32 * 4), ("w_off", C.c_int64 * 4), ("d_n", C.c_int32 * 4), ("d_off", C.c_int64 * 4),
+                ("comp_n", C.c_int32), ("n_active", C.c_int32), ("comp_off", C.c_int64), ("active", C.c_int32 * 8),
+                ("eps2", C.c_double), ("E_res", C.c_double), ("E_C", C.c_double), ("min_brem", C.c_double),
+                ("min_ann", C.c_double), ("min_comp", C.c_double), ("min_muon_brem", C.c_double)]
+
+
+class DarkOut(C.Structure):
+    _fields_ = [("p", C.c_void_p), ("r", C.c_void_p), ("parent", C.c_void_p), ("meta", C.c_void_p),
+                ("capacity", C.c_int64)]
+
+
+class DarkStats(C.Structure):
+    _fields_ = [("n_entries", C.c_int64), ("n_vectors", C.c_int64), ("n_draws", C.c_int64), ("n_trials", C.c_int64),
+                ("n_no_sample", C.c_int64), ("n_zero_rate", C.c_int64), ("n_pe_fail", C.c_int64), ("pad", C.c_int64),
+                ("yield_", C.c_double * 4), ("ms_select", C.c_double), ("ms_draw", C.c_double),
+                ("ms_emit", C.c_double), ("launches", C.c_int64)]
+
+    def as_dict(self):
+        d = {f: (float(getattr(self, f)) if t is C.c_double else int(getattr(self, f)))
+             for f, t in self._fields_ if f not in ("yield_", "pad")}
+        d["yield"] = [float(v) for v in self.yield_]
+        return d
+
+
 class PetiteError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[petite {code}] {msg}")
@@ -98,6 +126,10 @@ def load():
     lib.petite_eval_points.restype = C.c_int
     lib.petite_run_showers.argtypes = [vp, vp, i64, u64, C.POINTER(Store), vp, C.POINTER(Stats), vp]
     lib.petite_run_showers.restype = C.c_int
+    lib.petite_dark_setup.argtypes = [vp, C.POINTER(DarkDesc), vp, i64]
+    lib.petite_dark_setup.restype = C.c_int
+    lib.petite_dark_reweight.argtypes = [vp, C.POINTER(Store), i64, u64, C.POINTER(DarkOut), C.POINTER(DarkStats), vp]
+    lib.petite_dark_reweight.restype = C.c_int
     _lib = lib
     return lib
 

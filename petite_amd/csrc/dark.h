@@ -1,0 +1,414 @@
+// Dark-vector production over a shower record (DarkShower.generate_dark_shower,
+// src/PETITE/dark_shower.py:806-849).
+//
+// k_dark_select reads each SM record once and, for every active process with
+// a positive weight (GetBSMWeights, 595-647), writes an entry: the weight, the
+// pair's dark stream, and the state produce_bsm_particle (721-804) reaches
+// before its draw: interaction energy from the drate table, multiple
+// scattering over the distance travelled and the energy lost.  Entries that
+// need draw_dark_sample go to the per-process draw regions; k_dark_emit turns
+// every entry into its vector (threshold shortcuts, bound-electron Compton,
+// radiative return, dark bremsstrahlung) and rotates it into the lab frame.
+//
+// Random numbers of one pair come from dsid = derive_sid(seed, sid, code,
+// PURPOSE_DARK): INTERACT block 0 (x drate bin, y MCS radius, z MCS azimuth
+// with its sign in bit 0, w kinematic azimuth), INTERACT block 1 (x the bound
+// electron's cos(theta), y the azimuth of its frame), BOUND_E blocks (pairs of
+// bound-electron momentum accept/reject uniforms) and the VEGAS candidates.
+#pragma once
+
+namespace dark {
+
+enum Family { F_BREM_E = 0, F_BREM_P = 1, F_MUBREM = 2, F_ANN = 3 };
+enum State { S_NONE = 0, S_DRAW = 1, S_DIRECT = 2 };
+constexpr int kPePairs = 1 << 16;
+
+struct Entries {
+  int32_t* parent;   // SM store slot
+  int32_t* flag;     // code | state << 8
+  double* weight;    // wg * parent weight
+  double* E;         // energy after the drate step (pf[0] of produce_bsm_particle)
+  float4* dir;       // unit direction of pf
+  uint64_t* dsid;    // dark stream
+  int64_t cap;
+};
+
+// numpy.interp on an ascending grid (slope form), 0 outside [x0, x_{n-1}]
+// (scipy interp1d with fill_value=0, dark_shower.py:447-452).
+__device__ inline double interp_fill0(const double* x, const double* y, int n, double E) {
+  if (n <= 0 || !(E >= x[0]) || E > x[n - 1]) return 0.0;
+  if (E == x[n - 1]) return y[n - 1];
+  int lo = 0, hi = n - 1;  // x[lo] <= E < x[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (x[mid] <= E) lo = mid; else hi = mid;
+  }
+  const double slope = (y[lo + 1] - y[lo]) / (x[lo + 1] - x[lo]);
+  return slope * (E - x[lo]) + y[lo];
+}
+
+// interpolate1d in log10-log10 with fill -20 (dark_shower.py:31-46, 294-309).
+__device__ inline double loglog(const double* lx, const double* ly, int n, double E) {
+  const double l = log10(E);
+  if (n <= 0 || l < lx[0] || l > lx[n - 1]) return 1.0e-20;
+  int lo = 0, hi = n - 1;
+  if (l == lx[n - 1]) return pow(10.0, ly[n - 1]);
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (lx[mid] <= l) lo = mid; else hi = mid;
+  }
+  const double slope = (ly[lo + 1] - ly[lo]) / (lx[lo + 1] - lx[lo]);
+  return pow(10.0, slope * (l - lx[lo]) + ly[lo]);
+}
+
+__device__ inline int family_of(int code, int pid) {
+  if (code == DARKBREM) return pid == 11 ? F_BREM_E : (pid == -11 ? F_BREM_P : -1);
+  if (code == DARKANN) return pid == -11 ? F_ANN : -1;
+  if (code == DARKMUONBREM) return (pid == 13 || pid == -13) ? F_MUBREM : -1;
+  return -1;
+}
+
+// GetBSMWeights without the kinetic-mixing factor applied by the caller.
+__device__ inline double bsm_weight(const DevTables& T, const PetiteDarkDesc& D, const double* b, int pid, int code,
+                                    double E0) {
+  if (code == DARKCOMP) {
+    if (pid != 22 || E0 < D.min_comp || E0 < T.d->min_calc[2]) return 0.0;
+    const double* lx = b + D.comp_off;
+    const double num = loglog(lx, lx + D.comp_n, D.comp_n, E0);
+    return D.eps2 * (num / (nsigma(T, PAIRPROD, E0) + nsigma(T, COMP, E0)));
+  }
+  const int f = family_of(code, pid);
+  if (f < 0) return 0.0;
+  const double thr = (code == DARKBREM) ? D.min_brem : (code == DARKANN ? D.min_ann : D.min_muon_brem);
+  if (E0 < thr) return 0.0;
+  const double* x = b + D.w_off[f];
+  return D.eps2 * interp_fill0(x, x + D.w_n[f], D.w_n[f], E0);
+}
+
+// numpy's sum of a short row: sequential below 8 terms, else eight partial
+// sums combined pairwise and the remainder added in order.
+__device__ inline double np_sum(const double* r, int n, int stride) {
+  if (n < 8) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += r[i * stride];
+    return s;
+  }
+  double s = ((r[0] + r[stride]) + (r[2 * stride] + r[3 * stride])) +
+             ((r[4 * stride] + r[5 * stride]) + (r[6 * stride] + r[7 * stride]));
+  for (int i = 8; i < n; ++i) s += r[i * stride];
+  return s;
+}
+
+// Interaction energy of produce_bsm_particle (dark_shower.py:741-756): the
+// saved energy Ei (the largest <= E0, clamped to the table), a bin drawn with
+// probability proportional to its rate, shifted by E0 - Ei.  Returns false
+// when the row's rates sum to zero (the reference returns None).
+__device__ inline bool drate_energy(const PetiteDarkDesc& D, const double* b, int f, double E0, double u,
+                                    double& E_int) {
+  const int n = D.d_n[f];
+  if (n <= 0) return false;
+  const double* Es = b + D.d_off[f];
+  const double* cnt = Es + n;
+  const double* rows = cnt + n;
+  int i;
+  if (E0 < Es[0]) {
+    i = 0;
+  } else if (E0 > Es[n - 1]) {
+    i = n - 1;
+  } else {
+    int lo = 0, hi = n;  // largest i with Es[i] <= E0
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (Es[mid] <= E0) lo = mid; else hi = mid;
+    }
+    i = lo;
+  }
+  const int c = (int)cnt[i];
+  const double* row = rows + (int64_t)i * 2 * PETITE_DARK_DRATE_BINS;
+  const double tot = np_sum(row + 1, c, 2);
+  if (!(tot != 0.0) || c <= 0) return false;
+  double cdf[PETITE_DARK_DRATE_BINS];
+  double acc = 0.0;
+  for (int k = 0; k < c; ++k) {
+    acc += row[2 * k + 1] / tot;
+    cdf[k] = acc;
+  }
+  int k = c - 1;
+  for (int j = 0; j < c; ++j) {
+    if (cdf[j] / acc > u) {
+      k = j;
+      break;
+    }
+  }
+  E_int = row[2 * k] + (E0 - Es[i]);
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_dark_select(DevTables T, PetiteDarkDesc D, const double* b, DevStore S,
+                                                     int64_t first, int64_t n, uint64_t seed, Entries Q,
+                                                     Requests R, unsigned long long* n_entries,
+                                                     unsigned long long* stats) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool live = t < n;
+  const int64_t i = first + t;
+  int pid = 0;
+  float4 p0 = make_float4(0.f, 0.f, 0.f, 0.f), pf = p0;
+  double wpar = 0.0;
+  uint64_t sid = 0;
+  if (live) {
+    pid = S.pid[i];
+    p0 = S.p0[i];
+    pf = S.pf[i];
+    wpar = (double)S.r0[i].w;
+    sid = S.sid[i];
+  }
+  const double E0 = (double)p0.x;
+  unsigned long long n_zero = 0;
+  for (int a = 0; a < D.n_active; ++a) {
+    const int code = D.active[a];  // wave-uniform
+    const double wg = live ? bsm_weight(T, D, b, pid, code, E0) : 0.0;
+    const bool want = wg > 0.0;
+    const unsigned long long m = __ballot(want);
+    if (!m) continue;
+    unsigned long long base = 0;
+    if (lane == __ffsll((long long)m) - 1) base = atomicAdd(n_entries, (unsigned long long)__popcll(m));
+    base = shfl_u64(base, __ffsll((long long)m) - 1);
+    int state = S_NONE;
+    int64_t e = -1;
+    double E = 0.0;
+    float nx = 0.f, ny = 0.f, nz = 0.f;
+    if (want) {
+      e = (int64_t)(base + __popcll(m & ((1ull << lane) - 1ull)));
+      const uint64_t dsid = derive_sid(seed, sid, (uint32_t)code, PURPOSE_DARK);
+      const int f = (code == DARKCOMP) ? -1 : family_of(code, pid);
+      const double m_part = pid_mass(pid);
+      state = S_DIRECT;
+      if (f >= 0) {
+        const U4 w0 = draw_block(seed, dsid, 0, PURPOSE_INTERACT);
+        double E_int = 0.0;
+        if (!drate_energy(D, b, f, E0, u01d(w0.x), E_int)) {
+          state = S_NONE;
+          ++n_zero;
+        } else {
+          // scatter the initial momentum over the distance (m_lepton = m_e,
+          // dark_shower.py:753-755), then lose E0 - E_int.  beta comes from the
+          // on-shell |p| = sqrt(E0^2 - m^2): a float32 record's |p| can round to E0.
+          const double dist = (E0 - E_int) / T.d->dEdx;
+          const double px = p0.y, py = p0.z, pz = p0.w;
+          const double pn = sqrt(px * px + py * py + pz * pz);
+          const double pon = sqrt(fmax(E0 * E0 - m_part * m_part, 0.0));
+          if (pn > 0.0 && pon > 0.0) {
+            nx = (float)(px / pn);
+            ny = (float)(py / pn);
+            nz = (float)(pz / pn);
+            mcs_rotate(T, nx, ny, nz, E0, pon, m_part, kMe, dist, w0.y, w0.z);
+          }
+          double Eu = E0 - (E0 - E_int);
+          if (Eu <= m_part) Eu = m_part;
+          if (!(sqrt(Eu * Eu - m_part * m_part) > 0.0)) nx = ny = nz = 0.0f;
+          E = Eu;
+        }
+      } else {
+        E = (double)pf.x;
+        const double pn = sqrt((double)pf.y * pf.y + (double)pf.z * pf.z + (double)pf.w * pf.w);
+        if (pn > 0.0) {
+          nx = (float)(pf.y / pn);
+          ny = (float)(pf.z / pn);
+          nz = (float)(pf.w / pn);
+        }
+      }
+      if (state == S_DIRECT) {
+        const bool shortcut = (code == DARKANN && E <= D.E_res) || (code == DARKCOMP && E <= D.E_C);
+        if (!shortcut) state = S_DRAW;
+      }
+      if (e < Q.cap) {
+        Q.parent[e] = (int32_t)i;
+        Q.flag[e] = code | (state << 8);
+        Q.weight[e] = wg * wpar;
+        Q.E[e] = E;
+        Q.dir[e] = make_float4(nx, ny, nz, 0.0f);
+        Q.dsid[e] = dsid;
+      }
+    }
+    // draw requests of this process, one atomic per wave
+    const bool drq = want && state == S_DRAW && e < Q.cap;
+    const unsigned long long dm = __ballot(drq);
+    if (dm) {
+      const int lead = __ffsll((long long)dm) - 1;
+      unsigned int rb = 0;
+      if (lane == lead) rb = atomicAdd(R.count + code, (unsigned int)__popcll(dm));
+      rb = __shfl(rb, lead);
+      if (drq) {
+        const int64_t k = (int64_t)code * R.cap + rb + __popcll(dm & ((1ull << lane) - 1ull));
+        R.slot[k] = (int32_t)e;
+        R.E[k] = E;
+      }
+    }
+  }
+  wave_stat(stats, 0, n_zero);
+}
+
+// draw_pe_sample (dark_shower.py:706-719): pe uniform in [0, 1 MeV) accepted
+// against the 1s momentum density normalised at its maximum.
+__device__ inline double electron_wf(double pe, double lam) {
+  const double d = pe * pe + lam * lam;
+  return 32.0 / kPi * (lam * lam * lam * lam * lam) * pe * pe / (d * d * d * d);
+}
+__device__ inline bool draw_pe(uint64_t seed, uint64_t dsid, double Zeff, double& pe) {
+  const double lam = kAlpha * Zeff * kMe;
+  const double c = electron_wf(lam / sqrt(3.0), lam);
+  for (int k = 0; k < kPePairs / 2; ++k) {
+    const U4 w = draw_block(seed, dsid, (uint32_t)k, PURPOSE_BOUND_E);
+    double x = 1.0e-3 * u01d(w.x);
+    if (u01d(w.y) < electron_wf(x, lam) / c) {
+      pe = x;
+      return true;
+    }
+    x = 1.0e-3 * u01d(w.z);
+    if (u01d(w.w) < electron_wf(x, lam) / c) {
+      pe = x;
+      return true;
+    }
+  }
+  return false;
+}
+
+// compton_fourvecs_boundelectron (kinematics.py:134-183): the vector.
+__device__ inline V4 kin_compton_bound(double Eg, double mV, double ct, double Pe, double cte, double u_ph,
+                                       double u_phie) {
+  const double m = kMe;
+  const double s = m * m + 2.0 * Eg * (sqrt(m * m + Pe * Pe) - cte * Pe);
+  const double rs = sqrt(s);
+  const double Ee = (s - mV * mV + m * m) / (2.0 * rs);
+  const double EV = (s + mV * mV - m * m) / (2.0 * rs);
+  const double pF = sqrt(Ee * Ee - m * m);
+  const double bnum = sqrt(Eg * Eg + 2.0 * cte * Eg * Pe + Pe * Pe);
+  const double b0 = bnum / (Eg + sqrt(m * m + Pe * Pe));
+  const double g0 = 1.0 / sqrt(1.0 - b0 * b0);
+  double sph, cph;
+  sincos(2.0 * kPi * u_ph, &sph, &cph);
+  const double st = sqrt(1.0 - ct * ct);
+  const double vx = pF * st * sph, vy = pF * st * cph, vz = b0 * g0 * EV - g0 * pF * ct;
+  const double ctz = (Eg + cte * Pe) / bnum;
+  const double stz = sqrt(1.0 - ctz * ctz);
+  double spe, cpe;
+  sincos(2.0 * kPi * u_phie, &spe, &cpe);
+  return V4{g0 * EV - b0 * g0 * pF * ct, ctz * cpe * vx - spe * vy + stz * cpe * vz,
+            ctz * spe * vx + cpe * vy + stz * spe * vz, -stz * vx + ctz * vz};
+}
+
+// radiative_return_fourvecs (kinematics.py:267-299) with the boost of
+// radiative_return.py:18-24 into the frame of the electron at rest.
+__device__ inline V4 kin_radiative_return(double E, double u, double mV) {
+  const double m = kMe;
+  const double s = 2.0 * m * (m + E);
+  const double beta = (2.0 * kAlpha / kPi) * (log(s / (m * m)) - 1.0);
+  const double umax = pow(1.0 - mV * mV / s, beta / 2.0);
+  const double x1 = 1.0 - pow(u * umax, 2.0 / beta);
+  const double x2 = mV * mV / (x1 * s);
+  const double rs = sqrt(s);
+  const double E1 = x1 * rs / 2.0, E2 = x2 * rs / 2.0;
+  const double v0 = E1 + E2, v3 = E1 - E2;
+  // p = (rs/2, 0, 0, -sqrt(s/4 - m^2))
+  const double p0 = rs / 2.0, p3 = -sqrt(s / 4.0 - m * m);
+  const double rsq = sqrt(p0 * p0 - p3 * p3);
+  const double w0 = (p0 * v0 - p3 * v3) / rsq;
+  const double c1 = (v0 + w0) / (rsq + p0);
+  return V4{w0, 0.0, 0.0, v3 - c1 * p3};
+}
+
+struct EmitOut {
+  float4* p;
+  float4* r;
+  int32_t* parent;
+  int32_t* meta;
+};
+
+__global__ __launch_bounds__(256) void k_dark_emit(DevTables T, PetiteDarkDesc D, DevStore S, Entries Q, int64_t n_e,
+                                                   const float* x_res, const int32_t* trials_res, EmitOut O,
+                                                   int64_t out_base, uint64_t seed, double* yield,
+                                                   unsigned long long* stats) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double wsum[4] = {0.0, 0.0, 0.0, 0.0};
+  unsigned long long n_vec = 0, n_pe_fail = 0;
+  if (e < n_e) {
+    const int flag = Q.flag[e];
+    const int code = flag & 0xFF;
+    const int state = flag >> 8;
+    const int32_t slot = Q.parent[e];
+    double w = Q.weight[e];
+    const double E = Q.E[e];
+    const double mV = T.d->mV;
+    bool ok = state != S_NONE;
+    V4 v{0.0, 0.0, 0.0, 0.0};
+    if (ok && state == S_DIRECT) {
+      if (code == DARKANN) {
+        v = V4{sqrt(E * E - kMe * kMe + mV * mV), 0.0, 0.0, sqrt(E * E - kMe * kMe)};
+      } else {
+        v = V4{sqrt(E * E + mV * mV), 0.0, 0.0, E};
+      }
+    } else if (ok) {
+      ok = trials_res[e] > 0;
+      if (ok) {
+        const float* xs = x_res + 4 * e;
+        const uint64_t dsid = Q.dsid[e];
+        const U4 w0 = draw_block(seed, dsid, 0, PURPOSE_INTERACT);
+        if (code == DARKCOMP) {
+          const U4 w1 = draw_block(seed, dsid, 1, PURPOSE_INTERACT);
+          double pe = 0.0;
+          if (!draw_pe(seed, dsid, T.d->Zeff, pe)) {
+            ok = false;
+            ++n_pe_fail;
+          } else {
+            const double c0 = -1.0 + 2.0 * u01d(w1.x);
+            const double s = kMe * kMe + 2.0 * E * (sqrt(kMe * kMe + pe * pe) - c0 * pe);
+            const double Ee = (s - mV * mV + kMe * kMe) / (2.0 * sqrt(s));
+            if (Ee < kMe) {
+              v = V4{mV, 0.0, 0.0, 0.0};
+              w = 0.0;
+            } else {
+              v = kin_compton_bound(E, mV, (double)xs[0], pe, c0, u01d(w0.w), u01d(w1.y));
+            }
+          }
+        } else if (code == DARKANN) {
+          v = kin_radiative_return(E, (double)xs[0], mV);
+        } else {
+          // l_to_lV_fourvecs (kinematics.py:43-68): the vector
+          const double wv = (double)xs[0] * E;
+          const double ct = 1.0 - pow(10.0, (double)xs[1]);
+          const double k = sqrt(wv * wv - mV * mV);
+          double sa, ca;
+          sincos(2.0 * kPi * u01d(w0.w), &sa, &ca);
+          const double st = sqrt(1.0 - ct * ct);
+          v = V4{wv, k * ca * st, k * sa * st, k * ct};
+        }
+      }
+    }
+    float4 pv = make_float4(0.f, 0.f, 0.f, 0.f), rv = pv;
+    if (ok) {
+      const float4 d = Q.dir[e];
+      double ox, oy, oz;
+      frame_apply_d((double)d.x, (double)d.y, (double)d.z, v, ox, oy, oz);
+      pv = make_float4((float)v.e, (float)ox, (float)oy, (float)oz);
+      const float4 rf = S.rf[slot];
+      rv = make_float4(rf.x, rf.y, rf.z, (float)w);
+      ++n_vec;
+      wsum[code - DARKBREM] = w;
+    }
+    const int64_t o = out_base + e;
+    O.p[o] = pv;
+    O.r[o] = rv;
+    O.parent[o] = slot;
+    O.meta[o] = code | (ok ? 0x100 : 0);
+  }
+  for (int k = 0; k < 4; ++k) {
+    double s = wsum[k];
+    for (int off = 32; off > 0; off >>= 1) s += shfl_f64(s, (threadIdx.x & 63) ^ off);
+    if ((threadIdx.x & 63) == 0 && s != 0.0) unsafeAtomicAdd(yield + k, s);
+  }
+  wave_stat(stats, 1, n_vec);
+  wave_stat(stats, 2, n_pe_fail);
+}
+
+}  // namespace dark

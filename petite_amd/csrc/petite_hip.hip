@@ -43,6 +43,8 @@ enum CounterIdx {
   C_REFILL = 0,               // [PETITE_NPROC] per-process draw refill counters
   C_TRANSPORT = 16,           // transport refill counter
   C_STORE = 17,               // particles written to the store
+  C_DARK_N = 20,              // dark entries written in the current pass
+  C_DARK_STATS = 21,          // [3] drate rows summing to zero, vectors written, bound-electron momentum failures
   C_STATS = 32,               // [ST_N] run statistics
   C_PTRIALS = 64,             // [PETITE_NPROC] candidates per process
   C_PDRAWS = 80,              // [PETITE_NPROC] draws per process
@@ -742,6 +744,8 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
   }
 }
 
+#include "dark.h"
+
 }  // namespace
 
 // ===================================================================== C ABI
@@ -767,7 +771,43 @@ struct PetiteCtx {
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int n_cu = 256;
   double tbar[PETITE_NPROC];  // running mean of candidates per draw, per process
+  // dark sector
+  bool has_dark = false;
+  PetiteDarkDesc dark{};
+  double* dark_blob = nullptr;
+  dark::Entries ent{};
+  float* dres_x = nullptr;
+  int32_t* dres_trials = nullptr;
+  float* dres_w = nullptr;
+  double* dyield = nullptr;
 };
+
+static void free_entries(PetiteCtx* c) {
+  for (void* p : {(void*)c->ent.parent, (void*)c->ent.flag, (void*)c->ent.weight, (void*)c->ent.E, (void*)c->ent.dir,
+                  (void*)c->ent.dsid, (void*)c->dres_x, (void*)c->dres_trials, (void*)c->dres_w})
+    if (p) (void)hipFree(p);
+  c->ent = dark::Entries{};
+  c->dres_x = nullptr;
+  c->dres_trials = nullptr;
+  c->dres_w = nullptr;
+}
+
+static int ensure_entries(PetiteCtx* c, int64_t cap) {
+  if (cap <= c->ent.cap) return PETITE_OK;
+  free_entries(c);
+  dark::Entries& q = c->ent;
+  HIP_TRY(hipMalloc(&q.parent, cap * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&q.flag, cap * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&q.weight, cap * sizeof(double)));
+  HIP_TRY(hipMalloc(&q.E, cap * sizeof(double)));
+  HIP_TRY(hipMalloc(&q.dir, cap * sizeof(float4)));
+  HIP_TRY(hipMalloc(&q.dsid, cap * sizeof(uint64_t)));
+  HIP_TRY(hipMalloc(&c->dres_x, cap * 4 * sizeof(float)));
+  HIP_TRY(hipMalloc(&c->dres_trials, cap * sizeof(int32_t)));
+  HIP_TRY(hipMalloc(&c->dres_w, cap * sizeof(float)));
+  q.cap = cap;
+  return PETITE_OK;
+}
 
 static int ensure_requests(PetiteCtx* c, int64_t cap) {
   if (cap <= c->req_cap) return PETITE_OK;
@@ -925,8 +965,9 @@ void petite_ctx_destroy(PetiteCtx* c) {
     if (e) (void)hipEventDestroy(e);
   for (void* p : {(void*)c->d64, (void*)c->d32, (void*)c->dmaxf, (void*)c->req_slot, (void*)c->req_E, (void*)c->res_x,
                   (void*)c->res_trials, (void*)c->res_w, (void*)c->req_count, (void*)c->counters, (void*)c->ddesc,
-                  (void*)c->dsig_lds})
+                  (void*)c->dsig_lds, (void*)c->dark_blob, (void*)c->dyield})
     if (p) (void)hipFree(p);
+  free_entries(c);
   delete c;
 }
 
@@ -1128,6 +1169,138 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   if (written > store->capacity)
     return fail(PETITE_ERR_CAPACITY, "petite_run_showers: store capacity exceeded; n_particles holds the size reached");
   if (hs[ST_NOSAMPLE]) return fail(PETITE_ERR_NO_SAMPLE, "petite_run_showers: No Sample Found within max_trials");
+  return PETITE_OK;
+}
+
+int petite_dark_setup(PetiteCtx* c, const PetiteDarkDesc* desc, const double* blob, int64_t n) {
+  if (!c || !desc || !blob || n <= 0) return fail(PETITE_ERR_ARG, "petite_dark_setup: bad arguments");
+  if (desc->n_active < 0 || desc->n_active > 8) return fail(PETITE_ERR_ARG, "petite_dark_setup: n_active outside 0..8");
+  for (int a = 0; a < desc->n_active; ++a) {
+    const int p = desc->active[a];
+    if (p < DARKBREM || p > DARKMUONBREM) return fail(PETITE_ERR_ARG, "petite_dark_setup: active process is not a dark process");
+    if (c->desc.grid[p].n_E <= 0) return fail(PETITE_ERR_ARG, "petite_dark_setup: active process has no grid in the context");
+  }
+  for (int f = 0; f < PETITE_DARK_FAMILIES; ++f) {
+    if (desc->w_n[f] < 0 || (desc->w_n[f] > 0 && (desc->w_off[f] < 0 || desc->w_off[f] + 2 * (int64_t)desc->w_n[f] > n)))
+      return fail(PETITE_ERR_ARG, "petite_dark_setup: weight table outside the blob");
+    const int64_t need = (int64_t)desc->d_n[f] * (2 + 2 * PETITE_DARK_DRATE_BINS);
+    if (desc->d_n[f] < 0 || (desc->d_n[f] > 0 && (desc->d_off[f] < 0 || desc->d_off[f] + need > n)))
+      return fail(PETITE_ERR_ARG, "petite_dark_setup: drate table outside the blob");
+    const double* cnt = blob + desc->d_off[f] + desc->d_n[f];
+    for (int i = 0; i < desc->d_n[f]; ++i)
+      if (!(cnt[i] >= 0.0 && cnt[i] <= PETITE_DARK_DRATE_BINS))
+        return fail(PETITE_ERR_ARG, "petite_dark_setup: drate bin count outside 0..10");
+  }
+  if (desc->comp_n < 0 || (desc->comp_n > 0 && (desc->comp_off < 0 || desc->comp_off + 2 * (int64_t)desc->comp_n > n)))
+    return fail(PETITE_ERR_ARG, "petite_dark_setup: DarkComp table outside the blob");
+  if (c->dark_blob) {
+    (void)hipFree(c->dark_blob);
+    c->dark_blob = nullptr;
+  }
+  HIP_TRY(hipMalloc(&c->dark_blob, n * sizeof(double)));
+  HIP_TRY(hipMemcpy(c->dark_blob, blob, n * sizeof(double), hipMemcpyHostToDevice));
+  if (!c->dyield) HIP_TRY(hipMalloc(&c->dyield, 4 * sizeof(double)));
+  c->dark = *desc;
+  c->has_dark = true;
+  return PETITE_OK;
+}
+
+int petite_dark_reweight(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, uint64_t seed, PetiteDarkOut* out,
+                         PetiteDarkStats* stats, PetiteStream stream) {
+  if (!c || !sm || n_sm < 0 || !out || !stats) return fail(PETITE_ERR_ARG, "petite_dark_reweight: bad arguments");
+  if (!c->has_dark) return fail(PETITE_ERR_ARG, "petite_dark_reweight: no dark tables (petite_dark_setup)");
+  if (n_sm > sm->capacity) return fail(PETITE_ERR_ARG, "petite_dark_reweight: n_sm above the store capacity");
+  std::memset(stats, 0, sizeof(*stats));
+  if (n_sm == 0 || c->dark.n_active == 0) return PETITE_OK;
+  hipStream_t s = (hipStream_t)stream;
+  DevStore S{(float4*)sm->p0, (float4*)sm->r0, (float4*)sm->pf, (float4*)sm->rf, sm->pid, sm->parent, sm->meta,
+             sm->shower, sm->sid, sm->capacity};
+  const int na = c->dark.n_active;
+  // SM records per pass: entry scratch stays below 2^27 entries
+  const int64_t chunk = std::min<int64_t>(n_sm, std::max<int64_t>(1, ((int64_t)1 << 27) / na));
+  int rc = ensure_entries(c, chunk * na);
+  if (rc) return rc;
+  rc = ensure_requests(c, chunk * na);
+  if (rc) return rc;
+  unsigned long long* ctr = c->counters;
+  HIP_TRY(hipMemsetAsync(c->dyield, 0, 4 * sizeof(double), s));
+  HIP_TRY(hipMemsetAsync(ctr, 0, C_N * sizeof(unsigned long long), s));
+  int64_t out_base = 0;
+  double t_sel = 0.0, t_dr = 0.0, t_em = 0.0;
+  int64_t launches = 0;
+  for (int64_t first = 0; first < n_sm; first += chunk) {
+    const int64_t n = std::min(chunk, n_sm - first);
+    HIP_TRY(hipMemsetAsync(c->req_count, 0, PETITE_NPROC * sizeof(unsigned int), s));
+    HIP_TRY(hipMemsetAsync(ctr + C_DARK_N, 0, sizeof(unsigned long long), s));
+    HIP_TRY(hipMemsetAsync(ctr + C_REFILL, 0, PETITE_NPROC * sizeof(unsigned long long), s));  // draw refill
+    Requests R{c->req_slot, c->req_E, c->req_cap, c->req_count};
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], s));
+    hipLaunchKernelGGL(dark::k_dark_select, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c->dev, c->dark,
+                       (const double*)c->dark_blob, S, first, n, seed, c->ent, R, ctr + C_DARK_N, ctr + C_DARK_STATS);
+    HIP_TRY(hipGetLastError());
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], s));
+    unsigned int rq[PETITE_NPROC];
+    unsigned long long n_e = 0;
+    HIP_TRY(hipMemcpyAsync(rq, c->req_count, sizeof(rq), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&n_e, ctr + C_DARK_N, sizeof(n_e), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if ((int64_t)n_e > c->ent.cap) return fail(PETITE_ERR_HIP, "petite_dark_reweight: entry scratch overflow");
+    stats->n_entries += (int64_t)n_e;
+    if (out_base + (int64_t)n_e > out->capacity) {
+      return fail(PETITE_ERR_CAPACITY, "petite_dark_reweight: output smaller than the entries (n_entries holds the count so far)");
+    }
+    DrawLaunch io{};
+    io.req_slot = c->req_slot;
+    io.req_E = c->req_E;
+    io.sid_src = c->ent.dsid;
+    io.req_cap = c->req_cap;
+    io.explicit_out = 1;
+    io.x_out = c->dres_x;
+    io.trials_out = c->dres_trials;
+    io.w_out = c->dres_w;
+    io.counters = ctr;
+    int64_t blocks = 0;
+    plan_draw(c, rq, io, blocks);
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev[2], s));
+    rc = launch_draws(c, io, seed, s);
+    if (rc) return rc;
+    if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], s));
+    if (n_e > 0) {
+      dark::EmitOut O{(float4*)out->p, (float4*)out->r, out->parent, out->meta};
+      hipLaunchKernelGGL(dark::k_dark_emit, dim3((unsigned)((n_e + 255) / 256)), dim3(256), 0, s, c->dev, c->dark, S,
+                         c->ent, (int64_t)n_e, (const float*)c->dres_x, (const int32_t*)c->dres_trials, O, out_base,
+                         seed, c->dyield, ctr + C_DARK_STATS);
+      HIP_TRY(hipGetLastError());
+    }
+    if (c->timing) {
+      HIP_TRY(hipEventRecord(c->ev[4], s));
+      HIP_TRY(hipEventSynchronize(c->ev[4]));
+      float ms = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+      t_sel += ms;
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+      t_dr += ms;
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
+      t_em += ms;
+    }
+    ++launches;
+    out_base += (int64_t)n_e;
+  }
+  unsigned long long hs[ST_N], dst[3];
+  HIP_TRY(hipMemcpyAsync(hs, ctr + C_STATS, sizeof(hs), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(dst, ctr + C_DARK_STATS, sizeof(dst), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(stats->yield, c->dyield, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  stats->n_vectors = (int64_t)dst[1];
+  stats->n_zero_rate = (int64_t)dst[0];
+  stats->n_pe_fail = (int64_t)dst[2];
+  stats->n_draws = (int64_t)hs[ST_DRAWS];
+  stats->n_trials = (int64_t)hs[ST_TRIALS];
+  stats->n_no_sample = (int64_t)hs[ST_NOSAMPLE];
+  stats->ms_select = t_sel;
+  stats->ms_draw = t_dr;
+  stats->ms_emit = t_em;
+  stats->launches = launches;
   return PETITE_OK;
 }
 

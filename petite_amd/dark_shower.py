@@ -1,0 +1,257 @@
+"""DarkShower: drop-in for PETITE's ``DarkShower`` with the dark-vector
+production on the GPU.
+
+Same constructor and methods as src/PETITE/dark_shower.py:66-849 of the
+reference.  ``generate_dark_shower`` takes an SM shower (a list of Particle,
+a ParticleBatch, or a primary to shower first) and returns
+``(SM_shower, dark_vectors)``.  ``generate_dark_showers`` reweights a whole
+device-resident ParticleBatch in one call and returns a ``DarkBatch``.
+
+Random numbers: each (SM particle, dark process) pair draws from its own
+Philox stream derived from the particle's stream id and the process code, so
+the vectors do not depend on batch composition or launch shapes.  Particles
+passed as a list get stream id = their list position.
+"""
+import numpy as np
+import torch
+
+from .batch import ParticleBatch
+from .constants import all_process_code, alpha_em, dark_process_codes, mass_dict, PID_DARK_VECTOR
+from .dark_tables import DarkTables
+from .device import DeviceContext
+from .particle import Particle
+from .shower import Shower
+
+_DARK_NAMES = {8: "DarkBrem", 9: "DarkAnn", 10: "DarkComp", 11: "DarkMuonBrem"}
+_MESONS = (111, 221, 331)
+
+
+class DarkBatch:
+    """Device-resident dark vectors: one entry per (SM particle, active process)
+    with a positive weight; ``valid`` marks the entries that hold a vector
+    (produce_bsm_particle returned a Particle)."""
+
+    def __init__(self, out, n, stats, active, bound_electron=True):
+        self.n = int(n)
+        self._out = out
+        self.stats = stats
+        self.active = list(active)
+        self.bound_electron = bound_electron
+
+    def __len__(self):
+        return self.n
+
+    def tensor(self, key):
+        return self._out[key][: self.n]
+
+    @property
+    def p(self):
+        return self.tensor("p")
+
+    @property
+    def r(self):
+        return self.tensor("r")[:, :3]
+
+    @property
+    def weight(self):
+        return self.tensor("r")[:, 3]
+
+    @property
+    def parent(self):
+        return self.tensor("parent")
+
+    @property
+    def process(self):
+        return self.tensor("meta") & 0xFF
+
+    @property
+    def valid(self):
+        return (self.tensor("meta") & 0x100) != 0
+
+    @property
+    def yields(self):
+        """Sum of vector weights per process (float64, accumulated on the device)."""
+        return dict(zip(("DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"), self.stats["yield"]))
+
+    def order(self):
+        """Indices of the valid entries in the reference's order: SM record
+        order, then the active-process order."""
+        parent = self.parent.cpu().numpy().astype(np.int64)
+        proc = self.process.cpu().numpy()
+        rank = np.zeros(16, dtype=np.int64)
+        for i, c in enumerate(self.active):
+            rank[c] = i
+        ok = self.valid.cpu().numpy()
+        idx = np.nonzero(ok)[0]
+        return idx[np.lexsort((rank[proc[idx]], parent[idx]))]
+
+    def to_particles(self, sm_particles):
+        """Particle objects of the vectors (dark_shower.py:786-804); sm_particles
+        is the SM record as a list of Particle, indexed like the store slots."""
+        idx = self.order()
+        p = self.p.cpu().numpy().astype(np.float64)
+        r = self.tensor("r").cpu().numpy().astype(np.float64)
+        parent = self.parent.cpu().numpy()
+        proc = self.process.cpu().numpy()
+        out = []
+        for e in idx:
+            par = sm_particles[int(parent[e])].get_ids()
+            name = _DARK_NAMES[int(proc[e])]
+            if self.bound_electron and name in ("DarkAnn", "DarkComp"):
+                name += "_bound"
+            ids = {"PID": PID_DARK_VECTOR, "parent_PID": par["PID"], "ID": 2 * par["ID"] + 0,
+                   "parent_ID": par["ID"], "generation_number": par["generation_number"] + 1,
+                   "generation_process": name, "weight": float(r[e, 3])}
+            out.append(Particle(p[e], r[e, :3], ids))
+        return out
+
+
+class DarkShower(Shower):
+    """A class to reprocess an existing EM shower to generate dark photons
+    (src/PETITE/dark_shower.py:66)."""
+
+    def __init__(self, dict_dir, target_material, min_energy, mV_in_GeV, mode="exact", maxF_fudge_global=1,
+                 max_n_integrators=int(1e4), kinetic_mixing=1.0, Zeff=29.508, bound_electron=True, g_e=None,
+                 active_processes=None, fast_MCS_mode=True, rescale_MCS=1, seed=None, below="clamp"):
+        super().__init__(dict_dir, target_material, min_energy, maxF_fudge_global=maxF_fudge_global,
+                         max_n_integrators=max_n_integrators, fast_MCS_mode=fast_MCS_mode, seed=seed,
+                         rescale_MCS=rescale_MCS)
+        if mode not in ("exact", "approx"):
+            raise Exception("Mode not valid. Chose exact or approx.")
+        self.active_processes = list(dark_process_codes if active_processes is None else active_processes)
+        self.kinetic_mixing = kinetic_mixing
+        self.bound_electron = bound_electron
+        self.Zeff = Zeff
+        self.g_e = g_e if g_e is not None else kinetic_mixing * np.sqrt(4 * np.pi * alpha_em)
+        self._dark = DarkTables(self._tables, mV_in_GeV, mode=mode, Zeff=Zeff, bound_electron=bound_electron,
+                                kinetic_mixing=kinetic_mixing, below=below)
+        self._dark.eps2 = self.g_e ** 2 / (4 * np.pi * alpha_em)
+        self._mV = self._dark.mV
+        self._mV_estimator = self._dark.mV_est
+        self._mV_list = list(self._dark.mV_list)
+        self._resonant_annihilation_energy = self._dark.E_res
+        self._compton_threshold_energy = self._dark.E_C
+        self._minimum_calculable_dark_energy = self._dark.min_calc_dark
+
+    # ----------------------------------------------------------- accessors --
+    def get_mV(self):
+        return self._mV
+
+    def get_mV_list(self):
+        return self._mV_list
+
+    def get_DarkBremXSec(self):
+        return self._dark.xsec["DarkBrem"]
+
+    def get_DarkAnnXSec(self):
+        return self._dark.xsec["DarkAnn"]
+
+    def get_DarkCompXSec(self):
+        return self._dark.xsec["DarkComp"]
+
+    def get_DarkMuonBremXSec(self):
+        return self._dark.xsec["DarkMuonBrem"]
+
+    def _NSigmaDarkBrem(self, E):
+        return self._dark.ns["DarkBrem"](E)
+
+    def _NSigmaDarkAnn(self, E):
+        return self._dark.ns["DarkAnn"](E)
+
+    def _NSigmaDarkComp(self, E):
+        return self._dark.ns["DarkComp"](E)
+
+    def _NSigmaDarkMuonBrem(self, E):
+        return self._dark.ns["DarkMuonBrem"](E)
+
+    def _device_active(self):
+        return [all_process_code[p] for p in self.active_processes if p in all_process_code]
+
+    def device_context(self, MS=True):
+        key = bool(MS)
+        if key not in self._ctx:
+            ctx = DeviceContext(self._tables, self.min_energy, maxF_fudge=self._maxF_fudge_global,
+                                max_trials=int(self._max_n_integrators) * self._neval, MS=MS,
+                                rescale_MCS=self._MCS_rescale_factor, dark=self._dark)
+            ctx.dark_setup(self._device_active())
+            self._ctx[key] = ctx
+        return self._ctx[key]
+
+    # ------------------------------------------------------------- weights --
+    def GetBSMWeights(self, particle, process):
+        """Dark weight of one SM particle (dark_shower.py:595-647)."""
+        if isinstance(particle, (list, np.ndarray, tuple)):
+            PID, energy_initial = particle
+        else:
+            PID, energy_initial = particle.get_ids()["PID"], particle.get_p0()[0]
+        PID = int(PID)
+        if PID in _MESONS:
+            if process == "TwoBody_BSMDecay":
+                raise NotImplementedError("meson two-body decays (TwoBody_BSMDecay) are not implemented")
+            return 0.0
+        if process not in ("DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"):
+            return 0.0
+        return float(self._dark.bsm_weight(np.array([PID]), np.array([float(energy_initial)]), process)[0])
+
+    # ------------------------------------------------------------- records --
+    @staticmethod
+    def _store_from_particles(particles):
+        """Device store of a list of Particle (slot = list position, stream id = position)."""
+        n = len(particles)
+        dev = torch.device("cuda")
+        p0 = np.zeros((n, 4), dtype=np.float32)
+        pf = np.zeros((n, 4), dtype=np.float32)
+        r0 = np.zeros((n, 4), dtype=np.float32)
+        rf = np.zeros((n, 4), dtype=np.float32)
+        pid = np.zeros(n, dtype=np.int32)
+        for i, q in enumerate(particles):
+            ids = q.get_ids()
+            p0[i] = q.get_p0()
+            pf[i] = q.get_pf()
+            r0[i, :3] = q.get_r0()
+            r0[i, 3] = ids["weight"]
+            rf[i, :3] = q.get_rf()
+            pid[i] = ids["PID"]
+        t = lambda a: torch.from_numpy(a).to(dev)
+        return {"p0": t(p0), "pf": t(pf), "r0": t(r0), "rf": t(rf), "pid": t(pid),
+                "parent": torch.full((n,), -1, device=dev, dtype=torch.int32),
+                "meta": torch.zeros(n, device=dev, dtype=torch.int32),
+                "shower": torch.zeros(n, device=dev, dtype=torch.int32),
+                "sid": torch.arange(n, device=dev, dtype=torch.int64)}
+
+    def generate_dark_showers(self, record, seed=None, GlobalMS=True):
+        """Dark vectors of a whole SM record (ParticleBatch or list of Particle) as a DarkBatch."""
+        if isinstance(record, ParticleBatch):
+            store, n = record._store, record.n
+        else:
+            store, n = self._store_from_particles(record), len(record)
+            pids = set(int(v) for v in store["pid"].unique().cpu().numpy())
+            if pids & set(_MESONS) and "TwoBody_BSMDecay" in self.active_processes:
+                raise NotImplementedError("meson two-body decays (TwoBody_BSMDecay) are not implemented")
+        ctx = self.device_context(MS=GlobalMS)
+        out, n_e, stats = ctx.dark_reweight(store, n, self._seed if seed is None else seed)
+        return DarkBatch(out, n_e, stats, ctx.dark_active, self.bound_electron)
+
+    def generate_dark_shower(self, ExDir=None, SParams=None):
+        """(SM shower, dark vectors) of an existing shower or of a new one from
+        the primary SParams (dark_shower.py:806-849)."""
+        if ExDir is None and SParams is None:
+            print("Need an existing SM shower-file directory or SM incident particle to run dark shower")
+            return None
+        if ExDir is not None and isinstance(ExDir, str):
+            raise NotImplementedError("ExDir as a file path: the reference's pickled Particle lists are not "
+                                      "loaded; pass the list of Particle or a ParticleBatch")
+        if ExDir is not None and isinstance(ExDir, ParticleBatch):
+            dark = self.generate_dark_showers(ExDir)
+            sm = ExDir.to_particles()
+            return sm, dark.to_particles(sm)
+        if ExDir is not None and isinstance(ExDir, list):
+            sm = ExDir
+        elif isinstance(SParams, Particle):
+            sm = self.generate_shower(SParams)
+        else:
+            raise ValueError("Provided SParams must be a `Particle' class object")
+        if len(sm) == 0:
+            return sm, []
+        dark = self.generate_dark_showers(sm)
+        return sm, dark.to_particles(sm)

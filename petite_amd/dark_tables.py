@@ -389,3 +389,54 @@ class DarkTables:
             W = self.W["muon_brem"]
             out = np.where(m, _lin0(W[:, 0], W[:, 1], E0), out)
         return self.eps2 * out
+
+
+def build_dark_blob(dt: DarkTables, active_codes):
+    """Pack the weight arrays, drate tables and the DarkComp n*sigma of a
+    DarkTables into (DarkDesc, float64 blob) for petite_dark_setup."""
+    from . import _lib
+    desc = _lib.DarkDesc()
+    parts, n = [], 0
+
+    def put(a):
+        nonlocal n
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        off = n
+        parts.append(a)
+        n += a.size
+        return off
+
+    nb = _lib.DARK_DRATE_BINS
+    for f, name in enumerate(_lib.DARK_FAMILIES):
+        W = np.asarray(dt.W[name], dtype=np.float64)
+        if len(W) > 1 and np.any(np.diff(W[:, 0]) <= 0):
+            raise ValueError(f"{name}: weight energies must ascend")
+        desc.w_n[f] = len(W)
+        desc.w_off[f] = put(np.concatenate([W[:, 0], W[:, 1]])) if len(W) else 0
+        Es, rows, cnt = dt.D[name]
+        Es = np.asarray(Es, dtype=np.float64)
+        order = np.argsort(Es, kind="stable")
+        rows = np.asarray(rows, dtype=np.float64)[order]
+        cnt = np.asarray(cnt)[order]
+        pad = np.zeros((len(Es), nb, 2))
+        k = min(nb, rows.shape[1])
+        pad[:, :k, :] = rows[:, :k, :]
+        desc.d_n[f] = len(Es)
+        desc.d_off[f] = put(np.concatenate([Es[order], np.minimum(cnt, nb).astype(np.float64), pad.ravel()]))
+    ll = dt.ns["DarkComp"]
+    desc.comp_n = len(ll.lx)
+    desc.comp_off = put(np.concatenate([ll.lx, ll.ly]))
+    codes = list(active_codes)
+    if len(codes) > 8:
+        raise ValueError("at most 8 active dark processes")
+    desc.n_active = len(codes)
+    for i, c in enumerate(codes):
+        desc.active[i] = int(c)
+    desc.eps2 = float(dt.eps2)
+    desc.E_res, desc.E_C = float(dt.E_res), float(dt.E_C)
+    desc.min_brem = float(dt.min_calc_dark[11]["DarkBrem"])
+    desc.min_ann = float(dt.min_calc_dark[-11]["DarkAnn"])
+    desc.min_comp = float(dt.min_calc_dark[22]["DarkComp"])
+    desc.min_muon_brem = float(dt.min_calc_dark[13]["DarkMuonBrem"])
+    blob = np.concatenate(parts) if parts else np.zeros(1)
+    return desc, blob

@@ -158,3 +158,35 @@ class DeviceContext:
             if rc not in (_lib.PETITE_OK, _lib.PETITE_ERR_NO_SAMPLE):
                 _lib.check(rc, "petite_run_showers")
             return store, int(st.n_particles), st.as_dict(), energy, rc
+
+    # ------------------------------------------------------------ dark sector --
+    def dark_setup(self, active_codes):
+        """Upload the dark tables of the context's DarkTables for the given process codes."""
+        from .dark_tables import build_dark_blob
+        if self.dark is None:
+            raise ValueError("dark_setup: the context was built without dark tables")
+        desc, blob = build_dark_blob(self.dark, active_codes)
+        self._dark_desc, self._dark_blob = desc, blob
+        _lib.check(self.lib.petite_dark_setup(self.ctx, C.byref(desc), blob.ctypes.data, blob.size),
+                   "petite_dark_setup")
+        self.dark_active = list(active_codes)
+
+    def dark_reweight(self, store, n_sm, seed, capacity=None):
+        """Dark vectors of the first n_sm records of a particle store (dict of device tensors)."""
+        dev = torch.device("cuda")
+        n_sm = int(n_sm)
+        if capacity is None:
+            capacity = max(1, n_sm * max(1, len(self.dark_active)))
+        out = {"p": torch.empty((capacity, 4), device=dev, dtype=torch.float32),
+               "r": torch.empty((capacity, 4), device=dev, dtype=torch.float32),
+               "parent": torch.empty(capacity, device=dev, dtype=torch.int32),
+               "meta": torch.empty(capacity, device=dev, dtype=torch.int32)}
+        S = _lib.Store(*(store[k].data_ptr() for k in ("p0", "r0", "pf", "rf", "pid", "parent", "meta", "shower",
+                                                        "sid")), int(store["pid"].shape[0]))
+        O = _lib.DarkOut(out["p"].data_ptr(), out["r"].data_ptr(), out["parent"].data_ptr(), out["meta"].data_ptr(),
+                         capacity)
+        st = _lib.DarkStats()
+        _lib.check(self.lib.petite_dark_reweight(self.ctx, C.byref(S), n_sm, int(seed), C.byref(O), C.byref(st),
+                                                 _stream_handle()), "petite_dark_reweight")
+        return out, int(st.n_entries), st.as_dict()
+

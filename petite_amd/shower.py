@@ -239,7 +239,9 @@ class Shower:
         E, px, py, pz = [float(v) for v in np.asarray(p0.get_p0(), dtype=np.float64)]
         r = np.asarray(p0.get_r0(), dtype=np.float64)
         m_ids = p0.get_ids()["mass"]
-        if pid in _CHARGED and (m_ids is None or abs(float(m_ids) - m_ref) > 1e-6 * max(m_ref, 1e-12)):
+        # the reference rounds an inferred mass to 6 decimals (particle.py:125-131): only a
+        # larger mismatch (e.g. a massless charged primary) is reported
+        if pid in _CHARGED and (m_ids is None or abs(float(m_ids) - m_ref) > 1e-6):
             warnings.warn(f"primary PID {pid} has mass {m_ids}; transporting it on shell with m = {m_ref} "
                           "(the reference's multiple scattering is NaN for a massless charged lepton)")
             pn = np.sqrt(px * px + py * py + pz * pz)

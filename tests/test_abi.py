@@ -31,9 +31,11 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_c(tmp_path):
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "petite_hip.h"\n'
-                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(PetiteTableDesc), '
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(PetiteTableDesc), '
                    'sizeof(PetitePrimary), sizeof(PetiteStore), sizeof(PetiteStats), offsetof(PetiteTableDesc, grid), '
-                   'sizeof(PetiteGridDesc), sizeof(PetiteSigmaDesc)); return 0;}\n')
+                   'sizeof(PetiteGridDesc), sizeof(PetiteSigmaDesc), sizeof(PetiteDarkDesc), '
+                   'offsetof(PetiteDarkDesc, eps2), sizeof(PetiteDarkOut), sizeof(PetiteDarkStats), '
+                   'offsetof(PetiteDarkStats, yield)); return 0;}\n')
     exe = tmp_path / "sz"
     try:
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
@@ -42,7 +44,8 @@ def test_struct_layouts_match_c(tmp_path):
     got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
     want = [ctypes.sizeof(_lib.TableDesc), ctypes.sizeof(_lib.Primary), ctypes.sizeof(_lib.Store),
             ctypes.sizeof(_lib.Stats), _lib.TableDesc.grid.offset, ctypes.sizeof(_lib.GridDesc),
-            ctypes.sizeof(_lib.SigmaDesc)]
+            ctypes.sizeof(_lib.SigmaDesc), ctypes.sizeof(_lib.DarkDesc), _lib.DarkDesc.eps2.offset,
+            ctypes.sizeof(_lib.DarkOut), ctypes.sizeof(_lib.DarkStats), _lib.DarkStats.yield_.offset]
     assert got == want
 
 

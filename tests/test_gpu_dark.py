@@ -1,0 +1,139 @@
+"""GPU dark-vector production against the oracle (GetBSMWeights,
+produce_bsm_particle, draw_dark_sample; src/PETITE/dark_shower.py:595-804).
+
+The SM record comes from a device shower run; the oracle replays each sampled
+(SM particle, process) pair from the same record values, tables and dark
+stream, so entries agree one to one up to rare float32/float64 decision flips
+(MCS and the draw's accept test).  Tolerances: weights 1e-5 relative (float32
+output); vector components 5e-5 of the vector's energy."""
+import math
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+from oracle.dark import DARK_CODE, OracleDark, OracleDarkSampler
+from oracle.tables import SMTables
+from petite_amd import Particle
+from petite_amd.constants import mass_dict
+from petite_amd.dark_shower import DarkShower
+
+MAT, MV, SEED = "graphite", 0.03, 21
+PROCS = ["DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"]
+
+
+def _prim(E, pid):
+    m = mass_dict[pid]
+    return Particle([E, 0.0, 0.0, math.sqrt(E * E - m * m)], [0, 0, 0], {"PID": pid})
+
+
+@pytest.fixture(scope="module")
+def run():
+    ds = DarkShower("100GeV", MAT, 0.010, MV, seed=SEED)
+    rng = np.random.default_rng(3)
+    pids = [11, -11, 22] * 6 + [13, -13]
+    prims = [_prim(float(np.exp(rng.uniform(np.log(0.3), np.log(8.0)))) * (4 if abs(p) == 13 else 1), p)
+             for p in pids]
+    batch = ds.generate_showers(prims)
+    dark = ds.generate_dark_showers(batch)
+    ctx = ds.device_context()
+    maxf = {P: ctx.maxf[P] for P in PROCS}
+    od = OracleDark(SMTables(MAT), MV)
+    sampler = OracleDarkSampler(od, ds._dark.W, ds._dark.D, maxf, max_trials=ctx.desc.max_trials)
+    return ds, batch, dark, sampler
+
+
+def _records(a, idx):
+    out = []
+    for i in idx:
+        pid = int(a["pid"][i])
+        out.append({"pid": pid, "mass": mass_dict[pid], "p0": a["p0"][i].astype(np.float64),
+                    "pf": a["pf"][i].astype(np.float64), "rf": a["rf"][i][:3].astype(np.float64),
+                    "weight": float(a["r0"][i][3]), "sid": np.uint64(a["sid"][i])})
+    return out
+
+
+def test_entries_match_oracle(run):
+    ds, batch, dark, sampler = run
+    a = batch.to_numpy()
+    rng = np.random.default_rng(5)
+    idx = rng.choice(batch.n, size=min(batch.n, 400), replace=False)
+    recs = _records(a, idx)
+    dev = {}
+    par = dark.parent.cpu().numpy()
+    proc = dark.process.cpu().numpy()
+    val = dark.valid.cpu().numpy()
+    p = dark.p.cpu().numpy().astype(np.float64)
+    r = dark.tensor("r").cpu().numpy().astype(np.float64)
+    for e in range(dark.n):
+        dev[(int(par[e]), int(proc[e]))] = e
+    n_pairs = n_match = n_vec = 0
+    for slot, rec in zip(idx, recs):
+        for P in PROCS:
+            wo = sampler.weight(rec["pid"], rec["p0"][0], P)
+            e = dev.get((int(slot), DARK_CODE[P]))
+            assert (e is not None) == (wo > 0.0), (slot, P, wo)
+            if e is None:
+                continue
+            n_pairs += 1
+            v = sampler.produce(rec, P, wo, SEED)
+            if (v is not None) != bool(val[e]):
+                continue
+            if v is None:
+                n_match += 1
+                continue
+            n_vec += 1
+            ok_w = abs(r[e, 3] - v["weight"]) <= 1e-5 * abs(v["weight"]) + 1e-30
+            ok_p = np.max(np.abs(p[e] - v["p"])) <= 5e-5 * abs(v["p"][0]) + 1e-7
+            ok_r = np.allclose(r[e, :3], v["r"], rtol=1e-6, atol=1e-7)
+            n_match += ok_w and ok_p and ok_r
+    assert n_pairs > 100 and n_vec > 50, (n_pairs, n_vec)
+    assert n_match >= 0.98 * n_pairs, (n_match, n_pairs)
+
+
+def test_entry_count_and_weights(run):
+    ds, batch, dark, _ = run
+    a = batch.to_numpy()
+    # every entry corresponds to a positive host-side GetBSMWeights
+    par = dark.parent.cpu().numpy()
+    proc = dark.process.cpu().numpy()
+    w = dark.weight.cpu().numpy().astype(np.float64)
+    val = dark.valid.cpu().numpy()
+    names = {8: "DarkBrem", 9: "DarkAnn", 10: "DarkComp", 11: "DarkMuonBrem"}
+    for e in range(0, dark.n, max(1, dark.n // 300)):
+        i = int(par[e])
+        wg = ds.GetBSMWeights([int(a["pid"][i]), float(a["p0"][i][0])], names[int(proc[e])])
+        if val[e] and w[e] != 0.0:
+            assert abs(w[e] - wg * float(a["r0"][i][3])) <= 1e-5 * wg + 1e-30
+
+
+def test_vectors_on_shell_and_yields(run):
+    ds, batch, dark, _ = run
+    ok = dark.valid
+    p = dark.p[ok].double()
+    m2 = p[:, 0] ** 2 - (p[:, 1:] ** 2).sum(1)
+    assert torch.all((m2 - MV ** 2).abs() <= 2e-6 * p[:, 0] ** 2 + 1e-7)
+    assert torch.all(p[:, 0] >= MV * (1 - 1e-6))
+    w = dark.weight.double()
+    proc = dark.process
+    for k, P in enumerate(PROCS):
+        s = float(w[ok & (proc == 8 + k)].sum())
+        y = dark.yields[P]
+        assert abs(s - y) <= 1e-5 * abs(y) + 1e-12, (P, s, y)
+    assert dark.stats["n_no_sample"] == 0 and dark.stats["n_pe_fail"] == 0
+
+
+def test_drop_in_lists(run):
+    ds, _, _, _ = run
+    sm, vecs = ds.generate_dark_shower(SParams=_prim(3.0, 11))
+    assert len(sm) > 10 and len(vecs) > 0
+    ids = {q.get_ids()["ID"] for q in sm}
+    for v in vecs:
+        d = v.get_ids()
+        assert d["PID"] == 4900022
+        assert d["parent_ID"] in ids and d["ID"] == 2 * d["parent_ID"]
+        assert d["generation_process"] in ("DarkBrem", "DarkAnn_bound", "DarkComp_bound", "DarkMuonBrem")
+    sm2, vecs2 = ds.generate_dark_shower(ExDir=sm)
+    assert sm2 is sm and len(vecs2) == len(vecs)

@@ -1,0 +1,126 @@
+"""Oracle of the dark-vector production (produce_bsm_particle and helpers,
+src/PETITE/dark_shower.py:706-804, kinematics.py:43-68, 134-183, 267-299),
+pinned by invariants and reductions rather than by retyping:
+
+* the drate bin choice reproduces the row's probabilities exactly on a fine
+  grid of uniforms;
+* the bound-electron momentum draw follows the 1s momentum density (KS);
+* the bound-electron Compton vector at Pe = 0 equals the free Compton vector;
+* every vector is on shell (m_V), and the threshold shortcuts follow the
+  parent direction;
+* the sampler's weights equal the product's GetBSMWeights.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+from scipy.integrate import quad
+
+from oracle import kinematics, vegas
+from oracle.constants import alpha_em, m_electron, m_muon
+from oracle.dark import (DARK_CODE, OracleDark, OracleDarkSampler, draw_pe, drate_pick,
+                         electron_wave_function)
+from oracle.tables import SMTables
+from petite_amd.dark_tables import DarkTables
+from petite_amd.tables import MaterialTables
+
+MAT, MV = "graphite", 0.03
+
+
+@pytest.fixture(scope="module")
+def sampler():
+    dt = DarkTables(MaterialTables(MAT), MV)
+    T = SMTables(MAT)
+    od = OracleDark(T, MV)
+    maxf = {}
+    for P in DARK_CODE:
+        g = od._grids[P]
+        mx = []
+        for ie in range(len(g.E)):
+            ev = T.event_info(P, g.E[ie], od.mV)
+            mx.append(vegas.scan_maxf(g, P, ie, ev, n=4096)[0] * 1.5)
+        maxf[P] = np.array(mx)
+    return OracleDarkSampler(od, dt.W, dt.D, maxf), dt
+
+
+def test_drate_pick_reproduces_probabilities(sampler):
+    _, dt = sampler
+    Es, rows, cnt = dt.D["brem_elec"]
+    for i in (5, len(Es) // 2, len(Es) - 3):
+        E0 = Es[i] * (1 + 1e-9)
+        u = (np.arange(200000) + 0.5) / 200000
+        picks = np.array([drate_pick(Es, rows, cnt, E0, x) for x in u[::50]])
+        shift = E0 - Es[i]
+        centres = rows[i, :cnt[i], 0] + shift
+        frac = np.array([np.mean(np.isclose(picks, c, rtol=0, atol=1e-12)) for c in centres])
+        p = rows[i, :cnt[i], 1] / rows[i, :cnt[i], 1].sum()
+        assert np.max(np.abs(frac - p)) < 1e-3
+
+
+def test_pe_follows_1s_density():
+    Zeff = 29.508
+    pes = np.array([draw_pe(7, np.uint64(1000 + k), Zeff) for k in range(3000)])
+    lam = alpha_em * Zeff * m_electron
+    norm = quad(electron_wave_function, 0, 1e-3, args=(Zeff,), points=[lam])[0]
+    grid = np.linspace(0, 1e-3, 4001)
+    cdfv = np.concatenate([[0], np.cumsum([quad(electron_wave_function, a, b, args=(Zeff,))[0]
+                                           for a, b in zip(grid[:-1], grid[1:])])]) / norm
+    res = stats.kstest(pes, lambda x: np.interp(x, grid, cdfv))
+    assert res.pvalue > 1e-3
+
+
+def test_bound_compton_reduces_to_free_at_rest():
+    for E, ct, u in ((0.2, 0.3, 0.1), (3.0, -0.7, 0.8), (40.0, 0.99, 0.45)):
+        v = kinematics.compton_bound(E, [ct], MV, 0.0, 0.0, u, 0.0)
+        free = kinematics.compton(E, [ct], u, MV)[1]
+        np.testing.assert_allclose(v, free, rtol=1e-9, atol=1e-12)
+
+
+def _rec(pid, E, frac=1.0, sid=0):
+    m = {11: m_electron, -11: m_electron, 22: 0.0, 13: m_muon, -13: m_muon}[pid]
+    E = max(E, 2 * m)
+    p = math.sqrt(E * E - m * m)
+    Ef = E * frac
+    pf = math.sqrt(max(Ef * Ef - m * m, 0.0))
+    return {"pid": pid, "mass": m, "p0": np.array([E, 0.0, 0.0, p]),
+            "pf": np.array([E, 0, 0, p]) if pid == 22 else np.array([Ef, 0.0, pf * 0.6, pf * 0.8]),
+            "rf": np.array([0.0, 0.0, 0.05]), "weight": 0.5, "sid": np.uint64(sid)}
+
+
+def test_vectors_on_shell_and_weights(sampler):
+    s, dt = sampler
+    rng = np.random.default_rng(2)
+    recs = []
+    for i in range(60):
+        pid = (11, -11, 22, 13)[i % 4]
+        recs.append(_rec(pid, float(np.exp(rng.uniform(np.log(0.05), np.log(60.0)))), 0.7, 100 + i))
+    out = s.generate(recs, list(DARK_CODE), seed=3)
+    assert len(out) > 40
+    for v in out:
+        p = v["p"]
+        assert abs(p[0] ** 2 - p[1:] @ p[1:] - MV ** 2) < 1e-9 * p[0] ** 2 + 1e-12
+        rec = recs[v["parent"]]
+        P = v["process"].replace("_bound", "")
+        wg = s.weight(rec["pid"], rec["p0"][0], P)
+        assert v["weight"] in (0.0, wg * rec["weight"])
+        host = dt.bsm_weight(np.array([rec["pid"]]), np.array([rec["p0"][0]]), P)[0]
+        assert abs(host - wg) <= 1e-12 * abs(wg)
+        if P == "DarkBrem" or P == "DarkMuonBrem":
+            assert p[0] < rec["p0"][0]
+        else:
+            # the threshold shortcuts put the vector on shell at |p| of the parent
+            assert p[0] <= math.sqrt(rec["p0"][0] ** 2 + MV ** 2) + 2 * m_electron
+
+
+def test_threshold_shortcuts_follow_parent(sampler):
+    s, _ = sampler
+    od = s.od
+    # a photon just above the DarkComp weight threshold but below E_C: V along the photon
+    E = 0.5 * (od.E_C + max(od.min_dark[22]["DarkComp"], s.T.min_calc[22]))
+    rec = _rec(22, E, sid=9)
+    rec["p0"] = rec["pf"] = np.array([E, E * 0.6, 0.0, E * 0.8])
+    wg = s.weight(22, E, "DarkComp")
+    if wg > 0:
+        v = s.produce(rec, "DarkComp", wg, 1)
+        np.testing.assert_allclose(v["p"], [math.sqrt(E * E + MV ** 2), E * 0.6, 0.0, E * 0.8], rtol=1e-12)
