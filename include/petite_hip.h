@@ -226,6 +226,28 @@ int petite_dark_setup(PetiteCtx* ctx, const PetiteDarkDesc* desc, const double* 
 int petite_dark_reweight(PetiteCtx* ctx, const PetiteStore* sm, int64_t n_sm, uint64_t seed, PetiteDarkOut* out,
                          PetiteDarkStats* stats, PetiteStream stream);
 
+/* ------------------------------------------------- m_V-continuous yield --
+ * No counterpart in the reference, which only evaluates the trained masses.
+ * The yield of a shower record per unit eps^2 is
+ *   Y(m_V) = sum_i w_i sum_P W_P(E0_i; m_V),
+ * W_P = GetBSMWeights (src/PETITE/dark_shower.py:595-647) with eps^2 = 1.  For
+ * m_V between trained masses m_j < m_{j+1}, with t = ln(m_V/m_j)/ln(m_{j+1}/m_j),
+ * each particle's weight is W_j^(1-t) W_{j+1}^t when both are positive and
+ * (1-t) W_j + t W_{j+1} otherwise; t is clamped to [0, 1] outside the trained
+ * range (there dY/dm_V = 0). */
+#define PETITE_MAX_MASSES 8
+
+/* Dark tables of n_mass trained masses (ascending), one descriptor each; all
+ * offsets index the one float64 blob (host, n doubles).  descs[j].eps2 should
+ * be 1.  The context must hold the SM n*sigma tables of the same material. */
+int petite_yield_setup(PetiteCtx* ctx, int32_t n_mass, const double* masses, const PetiteDarkDesc* descs,
+                       const double* blob, int64_t n);
+
+/* Y and dY/dm_V at n_points masses (host arrays), accumulated in float64 over
+ * the first n_sm records of the store. */
+int petite_dark_yield(PetiteCtx* ctx, const PetiteStore* sm, int64_t n_sm, int32_t n_points, const double* mV_host,
+                      double* yield_host, double* dyield_dmV_host, PetiteStream stream);
+
 #ifdef __cplusplus
 }
 #endif

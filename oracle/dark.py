@@ -171,7 +171,7 @@ class LogInterp:
 
 class OracleDark:
     def __init__(self, sm_tables, mV_in_GeV, mode="exact", Zeff=29.508, kinetic_mixing=1.0, table_set="100GeV",
-                 below="clamp", use_cache=True):
+                 below="clamp", use_cache=True, with_integrals=True):
         z, meta = load_pack("dark", table_set)
         self.T = sm_tables
         T = sm_tables
@@ -214,7 +214,7 @@ class OracleDark:
                    "DarkMuonBrem": LogInterp(DMB[:, 0], nZ * GeVsqcm2 * DMB[:, 1])}
         # interaction integrals via quad of the interpolants (shower.py:298-320)
         self.II = {}
-        for P in ("Brem", "Ann", "Moller", "Bhabha", "MuonE"):
+        for P in (("Brem", "Ann", "Moller", "Bhabha", "MuonE") if with_integrals else ()):
             xg = T.nsigma_tab[P][0]
             f = lambda e, P=P: float(T.nsigma(P, e))
             vals = np.array([quad(f, xg[0], xg[i], full_output=1, limit=200, points=xg[:i + 1][1:-1] if i > 1 else None)[0]
@@ -223,9 +223,10 @@ class OracleDark:
         xg = T.nsigma_tab["MuonBrem"][0]
         f = lambda e: float(T.nsigma("MuonE", e))
         mx = T.nsigma_tab["MuonE"][0]
-        self.II["MuonBrem"] = (xg, np.array([quad(f, xg[0], xg[i], full_output=1, limit=200,
-                                                  points=mx[(mx > xg[0]) & (mx < xg[i])])[0] if i > 0 else 0.0
-                                             for i in range(len(xg))]))
+        if with_integrals:
+            self.II["MuonBrem"] = (xg, np.array([quad(f, xg[0], xg[i], full_output=1, limit=200,
+                                                      points=mx[(mx > xg[0]) & (mx < xg[i])])[0] if i > 0 else 0.0
+                                                 for i in range(len(xg))]))
         self.dEdxT = T.dEdx
         self.z, self.key = z, key
         self.use_cache = use_cache
@@ -426,3 +427,64 @@ class OracleDarkSampler:
                         v["parent"] = i
                         out.append(v)
         return out
+
+
+# ------------------------------------------------------- m_V-continuous yield --
+def sampler_weights(s, pid, E0, process):
+    """OracleDarkSampler.weight over arrays of (pid, E0)."""
+    pid = np.asarray(pid)
+    E0 = np.asarray(E0, dtype=np.float64)
+    od, T = s.od, s.T
+    out = np.zeros(E0.shape)
+    if process == "DarkComp":
+        m = (pid == 22) & (E0 >= od.min_dark[22]["DarkComp"]) & (E0 >= T.min_calc[22])
+        if m.any():
+            e = E0[m]
+            out[m] = od.eps2 * od.NS["DarkComp"](e) / (T.nsigma("PairProd", e) + T.nsigma("Comp", e))
+        return out
+    for (P, p), name in W_FAMILY.items():
+        if P != process:
+            continue
+        m = (pid == p) & (E0 >= od.min_dark[p][process])
+        if m.any():
+            W = s.W[name]
+            out[m] = od.eps2 * interp0(W[:, 0], W[:, 1], E0[m])
+    return out
+
+
+def yield_and_grad(samplers, masses, processes, pid, E0, w, mV):
+    """Y(m_V) = sum_i w_i sum_P W_P(E0_i; m_V) and dY/dm_V with each particle's
+    weight interpolated between the trained masses: geometric in
+    t = ln(m_V/m_j)/ln(m_{j+1}/m_j) where both weights are positive, linear
+    otherwise; t clamped to [0, 1] (zero derivative) outside the masses.
+    samplers[j] evaluates the weights at masses[j] with eps^2 = 1."""
+    masses = np.asarray(masses, dtype=np.float64)
+    wts = np.array([[sampler_weights(samplers[j], pid, E0, P) for P in processes] for j in range(len(masses))])
+    w = np.asarray(w, dtype=np.float64)
+    Y, dY = [], []
+    for m in np.atleast_1d(mV):
+        j = 0
+        while j + 1 < len(masses) - 1 and masses[j + 1] <= m:
+            j += 1
+        if len(masses) == 1:
+            lo = hi = wts[0]
+            t = dtdm = 0.0
+        else:
+            dl = math.log(masses[j + 1] / masses[j])
+            t, dtdm = math.log(m / masses[j]) / dl, 1.0 / (m * dl)
+            if t <= 0.0:
+                t = 0.0
+                dtdm = 0.0 if m < masses[0] else dtdm
+            if t >= 1.0:
+                t = 1.0
+                dtdm = 0.0 if m > masses[-1] else dtdm
+            lo, hi = wts[j], wts[j + 1]
+        both = (lo > 0) & (hi > 0)
+        with np.errstate(all="ignore"):
+            r = np.where(both, np.log(hi / lo), 0.0)
+            Wg = lo * np.exp(t * r)
+        W = np.where(both, Wg, lo + t * (hi - lo))
+        dW = np.where(both, Wg * r * dtdm, (hi - lo) * dtdm)
+        Y.append(float(np.sum(W.sum(0) * w)))
+        dY.append(float(np.sum(dW.sum(0) * w)))
+    return np.array(Y), np.array(dY)

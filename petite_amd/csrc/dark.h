@@ -411,4 +411,80 @@ __global__ __launch_bounds__(256) void k_dark_emit(DevTables T, PetiteDarkDesc D
   wave_stat(stats, 2, n_pe_fail);
 }
 
+// ------------------------------------------------------ m_V-continuous yield --
+constexpr int kYieldBatch = 8;
+
+struct YieldTables {
+  PetiteDarkDesc d[PETITE_MAX_MASSES];
+  int32_t n_mass;
+  const double* b;
+};
+
+// Mass points of one pass: segment j (masses j, j+1), t and dt/dm_V.
+struct YieldPoints {
+  int32_t n;
+  int32_t j[kYieldBatch];
+  double t[kYieldBatch];
+  double dtdm[kYieldBatch];
+};
+
+// One float64 pass over the record: acc[2k] += w W(m_k), acc[2k+1] += w dW/dm(m_k).
+__global__ __launch_bounds__(256) void k_dark_yield(DevTables T, YieldTables Y, DevStore S, int64_t n, YieldPoints P,
+                                                    double* acc) {
+  double y[kYieldBatch], dy[kYieldBatch];
+#pragma unroll
+  for (int k = 0; k < kYieldBatch; ++k) y[k] = dy[k] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const PetiteDarkDesc& d0 = Y.d[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int pid = S.pid[i];
+    const double E0 = (double)S.p0[i].x;
+    const double w = (double)S.r0[i].w;
+    for (int a = 0; a < d0.n_active; ++a) {
+      const int code = d0.active[a];
+      if (code == DARKCOMP ? pid != 22 : family_of(code, pid) < 0) continue;
+      double wj[PETITE_MAX_MASSES];
+#pragma unroll
+      for (int j = 0; j < PETITE_MAX_MASSES; ++j)
+        wj[j] = j < Y.n_mass ? bsm_weight(T, Y.d[j], Y.b, pid, code, E0) : 0.0;
+#pragma unroll
+      for (int k = 0; k < kYieldBatch; ++k) {
+        if (k >= P.n) break;
+        const int j = P.j[k];
+        double lo = 0.0, hi = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < PETITE_MAX_MASSES; ++jj) {
+          lo = jj == j ? wj[jj] : lo;
+          hi = jj == j + 1 ? wj[jj] : hi;
+        }
+        if (j + 1 >= Y.n_mass) hi = lo;
+        const double t = P.t[k];
+        double W, dW;
+        if (lo > 0.0 && hi > 0.0) {
+          const double r = log(hi / lo);
+          W = lo * exp(t * r);
+          dW = W * r * P.dtdm[k];
+        } else {
+          W = lo + t * (hi - lo);
+          dW = (hi - lo) * P.dtdm[k];
+        }
+        y[k] += w * W;
+        dy[k] += w * dW;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kYieldBatch; ++k) {
+    double a = y[k], b = dy[k];
+    for (int off = 32; off > 0; off >>= 1) {
+      a += shfl_f64(a, (threadIdx.x & 63) ^ off);
+      b += shfl_f64(b, (threadIdx.x & 63) ^ off);
+    }
+    if ((threadIdx.x & 63) == 0 && k < P.n) {
+      unsafeAtomicAdd(acc + 2 * k, a);
+      unsafeAtomicAdd(acc + 2 * k + 1, b);
+    }
+  }
+}
+
 }  // namespace dark

@@ -780,6 +780,12 @@ struct PetiteCtx {
   int32_t* dres_trials = nullptr;
   float* dres_w = nullptr;
   double* dyield = nullptr;
+  // trained masses for the m_V-continuous yield
+  int32_t n_mass = 0;
+  double masses[PETITE_MAX_MASSES] = {};
+  dark::YieldTables ytab{};
+  double* yield_blob = nullptr;
+  double* yacc = nullptr;
 };
 
 static void free_entries(PetiteCtx* c) {
@@ -965,7 +971,7 @@ void petite_ctx_destroy(PetiteCtx* c) {
     if (e) (void)hipEventDestroy(e);
   for (void* p : {(void*)c->d64, (void*)c->d32, (void*)c->dmaxf, (void*)c->req_slot, (void*)c->req_E, (void*)c->res_x,
                   (void*)c->res_trials, (void*)c->res_w, (void*)c->req_count, (void*)c->counters, (void*)c->ddesc,
-                  (void*)c->dsig_lds, (void*)c->dark_blob, (void*)c->dyield})
+                  (void*)c->dsig_lds, (void*)c->dark_blob, (void*)c->dyield, (void*)c->yield_blob, (void*)c->yacc})
     if (p) (void)hipFree(p);
   free_entries(c);
   delete c;
@@ -1172,27 +1178,37 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   return PETITE_OK;
 }
 
-int petite_dark_setup(PetiteCtx* c, const PetiteDarkDesc* desc, const double* blob, int64_t n) {
-  if (!c || !desc || !blob || n <= 0) return fail(PETITE_ERR_ARG, "petite_dark_setup: bad arguments");
-  if (desc->n_active < 0 || desc->n_active > 8) return fail(PETITE_ERR_ARG, "petite_dark_setup: n_active outside 0..8");
-  for (int a = 0; a < desc->n_active; ++a) {
-    const int p = desc->active[a];
-    if (p < DARKBREM || p > DARKMUONBREM) return fail(PETITE_ERR_ARG, "petite_dark_setup: active process is not a dark process");
-    if (c->desc.grid[p].n_E <= 0) return fail(PETITE_ERR_ARG, "petite_dark_setup: active process has no grid in the context");
-  }
+static int check_dark_desc(const PetiteDarkDesc* desc, const double* blob, int64_t n, const char* who) {
+  if (desc->n_active < 0 || desc->n_active > 8) return fail(PETITE_ERR_ARG, std::string(who) + ": n_active outside 0..8");
+  for (int a = 0; a < desc->n_active; ++a)
+    if (desc->active[a] < DARKBREM || desc->active[a] > DARKMUONBREM)
+      return fail(PETITE_ERR_ARG, std::string(who) + ": active process is not a dark process");
   for (int f = 0; f < PETITE_DARK_FAMILIES; ++f) {
     if (desc->w_n[f] < 0 || (desc->w_n[f] > 0 && (desc->w_off[f] < 0 || desc->w_off[f] + 2 * (int64_t)desc->w_n[f] > n)))
-      return fail(PETITE_ERR_ARG, "petite_dark_setup: weight table outside the blob");
+      return fail(PETITE_ERR_ARG, std::string(who) + ": weight table outside the blob");
     const int64_t need = (int64_t)desc->d_n[f] * (2 + 2 * PETITE_DARK_DRATE_BINS);
     if (desc->d_n[f] < 0 || (desc->d_n[f] > 0 && (desc->d_off[f] < 0 || desc->d_off[f] + need > n)))
-      return fail(PETITE_ERR_ARG, "petite_dark_setup: drate table outside the blob");
+      return fail(PETITE_ERR_ARG, std::string(who) + ": drate table outside the blob");
     const double* cnt = blob + desc->d_off[f] + desc->d_n[f];
     for (int i = 0; i < desc->d_n[f]; ++i)
       if (!(cnt[i] >= 0.0 && cnt[i] <= PETITE_DARK_DRATE_BINS))
-        return fail(PETITE_ERR_ARG, "petite_dark_setup: drate bin count outside 0..10");
+        return fail(PETITE_ERR_ARG, std::string(who) + ": drate bin count outside 0..10");
   }
   if (desc->comp_n < 0 || (desc->comp_n > 0 && (desc->comp_off < 0 || desc->comp_off + 2 * (int64_t)desc->comp_n > n)))
-    return fail(PETITE_ERR_ARG, "petite_dark_setup: DarkComp table outside the blob");
+    return fail(PETITE_ERR_ARG, std::string(who) + ": DarkComp table outside the blob");
+  return PETITE_OK;
+}
+
+int petite_dark_setup(PetiteCtx* c, const PetiteDarkDesc* desc, const double* blob, int64_t n) {
+  if (!c || !desc || !blob || n <= 0) return fail(PETITE_ERR_ARG, "petite_dark_setup: bad arguments");
+  if (desc->n_active < 0 || desc->n_active > 8) return fail(PETITE_ERR_ARG, "petite_dark_setup: n_active outside 0..8");
+  for (int a = 0; a < desc->n_active; ++a)
+    if (desc->active[a] >= DARKBREM && desc->active[a] <= DARKMUONBREM && c->desc.grid[desc->active[a]].n_E <= 0)
+      return fail(PETITE_ERR_ARG, "petite_dark_setup: active process has no grid in the context");
+  {
+    const int rc = check_dark_desc(desc, blob, n, "petite_dark_setup");
+    if (rc) return rc;
+  }
   if (c->dark_blob) {
     (void)hipFree(c->dark_blob);
     c->dark_blob = nullptr;
@@ -1301,6 +1317,91 @@ int petite_dark_reweight(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, uint
   stats->ms_draw = t_dr;
   stats->ms_emit = t_em;
   stats->launches = launches;
+  return PETITE_OK;
+}
+
+int petite_yield_setup(PetiteCtx* c, int32_t n_mass, const double* masses, const PetiteDarkDesc* descs,
+                       const double* blob, int64_t n) {
+  if (!c || !masses || !descs || !blob || n <= 0 || n_mass < 1 || n_mass > PETITE_MAX_MASSES)
+    return fail(PETITE_ERR_ARG, "petite_yield_setup: bad arguments");
+  for (int j = 0; j < n_mass; ++j) {
+    if (!(masses[j] > 0.0) || (j > 0 && !(masses[j] > masses[j - 1])))
+      return fail(PETITE_ERR_ARG, "petite_yield_setup: masses must be positive and ascending");
+    int rc = check_dark_desc(descs + j, blob, n, "petite_yield_setup");
+    if (rc) return rc;
+    if (descs[j].n_active != descs[0].n_active ||
+        std::memcmp(descs[j].active, descs[0].active, sizeof(descs[0].active)) != 0)
+      return fail(PETITE_ERR_ARG, "petite_yield_setup: the masses must share the active processes");
+  }
+  if (c->yield_blob) {
+    (void)hipFree(c->yield_blob);
+    c->yield_blob = nullptr;
+  }
+  HIP_TRY(hipMalloc(&c->yield_blob, n * sizeof(double)));
+  HIP_TRY(hipMemcpy(c->yield_blob, blob, n * sizeof(double), hipMemcpyHostToDevice));
+  if (!c->yacc) HIP_TRY(hipMalloc(&c->yacc, 2 * dark::kYieldBatch * sizeof(double)));
+  c->n_mass = n_mass;
+  for (int j = 0; j < n_mass; ++j) {
+    c->masses[j] = masses[j];
+    c->ytab.d[j] = descs[j];
+  }
+  c->ytab.n_mass = n_mass;
+  c->ytab.b = c->yield_blob;
+  return PETITE_OK;
+}
+
+int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t n_points, const double* mV,
+                      double* yield_host, double* dyield_host, PetiteStream stream) {
+  if (!c || !sm || n_sm < 0 || n_points < 0 || (n_points > 0 && (!mV || !yield_host || !dyield_host)))
+    return fail(PETITE_ERR_ARG, "petite_dark_yield: bad arguments");
+  if (c->n_mass < 1) return fail(PETITE_ERR_ARG, "petite_dark_yield: no trained masses (petite_yield_setup)");
+  if (n_sm > sm->capacity) return fail(PETITE_ERR_ARG, "petite_dark_yield: n_sm above the store capacity");
+  for (int k = 0; k < n_points; ++k)
+    if (!(mV[k] > 0.0)) return fail(PETITE_ERR_ARG, "petite_dark_yield: m_V must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  DevStore S{(float4*)sm->p0, (float4*)sm->r0, (float4*)sm->pf, (float4*)sm->rf, sm->pid, sm->parent, sm->meta,
+             sm->shower, sm->sid, sm->capacity};
+  const int nm = c->n_mass;
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n_sm + 255) / 256, (int64_t)c->n_cu * 8));
+  for (int k0 = 0; k0 < n_points; k0 += dark::kYieldBatch) {
+    dark::YieldPoints P{};
+    P.n = std::min(dark::kYieldBatch, n_points - k0);
+    for (int k = 0; k < P.n; ++k) {
+      const double m = mV[k0 + k];
+      int j = 0;
+      while (j + 1 < nm - 1 && c->masses[j + 1] <= m) ++j;
+      double t = 0.0, dtdm = 0.0;
+      if (nm > 1) {
+        const double dl = std::log(c->masses[j + 1] / c->masses[j]);
+        t = std::log(m / c->masses[j]) / dl;
+        dtdm = 1.0 / (m * dl);
+        if (t <= 0.0) {
+          t = 0.0;
+          if (m < c->masses[0]) dtdm = 0.0;
+        }
+        if (t >= 1.0) {
+          t = 1.0;
+          if (m > c->masses[nm - 1]) dtdm = 0.0;
+        }
+      }
+      P.j[k] = j;
+      P.t[k] = t;
+      P.dtdm[k] = dtdm;
+    }
+    HIP_TRY(hipMemsetAsync(c->yacc, 0, 2 * dark::kYieldBatch * sizeof(double), s));
+    if (n_sm > 0) {
+      hipLaunchKernelGGL(dark::k_dark_yield, dim3((unsigned)blocks), dim3(256), 0, s, c->dev, c->ytab, S, n_sm, P,
+                         c->yacc);
+      HIP_TRY(hipGetLastError());
+    }
+    double h[2 * dark::kYieldBatch];
+    HIP_TRY(hipMemcpyAsync(h, c->yacc, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int k = 0; k < P.n; ++k) {
+      yield_host[k0 + k] = h[2 * k];
+      dyield_host[k0 + k] = h[2 * k + 1];
+    }
+  }
   return PETITE_OK;
 }
 

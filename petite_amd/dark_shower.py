@@ -177,6 +177,15 @@ class DarkShower(Shower):
             self._ctx[key] = ctx
         return self._ctx[key]
 
+    def dark_yield(self, record, mV, eps2=1.0):
+        """eps^2 * Y(m_V) of a shower record, differentiable in m_V and eps^2
+        (torch autograd; see petite_amd.dark_yield)."""
+        from .dark_yield import DarkYield
+        if getattr(self, "_yield_model", None) is None:
+            self._yield_model = DarkYield(self._tables, self.min_energy, active_processes=self.active_processes,
+                                          Zeff=self.Zeff, ctx=self.device_context())
+        return self._yield_model(record, mV, eps2)
+
     # ------------------------------------------------------------- weights --
     def GetBSMWeights(self, particle, process):
         """Dark weight of one SM particle (dark_shower.py:595-647)."""

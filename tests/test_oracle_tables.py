@@ -73,3 +73,54 @@ def test_lu_key_picks_next_higher(tabs):
     for ie in (3, 50, 98):
         assert g.lu_key(g.E[ie]) == ie + 1
     assert g.lu_key(1e6) == len(g.E) - 1
+
+
+# ------------------------------------------------------------------- dark --
+# Same identity for dark_xsec.pkl[mV][P][m], evaluated with the table's target
+# mass mT = target_information[m]["mT"] (utilities/find_maxes.py:88).  Floors:
+# DarkBrem / DarkMuonBrem at the spread of their 30,000-point tables (seeds of
+# the same estimator scatter by a few % for DarkBrem and up to ~10 % for
+# DarkMuonBrem at m_V >= 0.03; at 0.003 DarkMuonBrem is heavy tailed and not
+# tested); DarkComp is smooth (1e-3).  DarkAnn's radiative-return integrand is
+# heavy tailed near u -> 0: quadrature of the same integrand sits 20-40 %
+# above both the table and any 30,000-point estimate, so only a band is
+# checked there.
+DARK_FLOOR = {"DarkBrem": 0.05, "DarkMuonBrem": 0.15, "DarkComp": 2e-3}
+
+
+def _dark_moments(T, P, mv, ie, n=N):
+    from oracle.constants import target_information
+    from oracle.tables import Grid, load_pack
+    z, _ = load_pack("dark", "100GeV")
+    g = Grid(z[f"E__{P}__{mv}"], z[f"nodes__{P}__{mv}"], z[f"nodeoff__{P}__{mv}"], z[f"nodecnt__{P}__{mv}"])
+    ev = T.event_info(P, g.E[ie], float(mv))
+    ev["mT"] = target_information[T.material]["mT"]
+    sid = np.uint64((PROCESS_CODE[P] << 32) | ie)
+    y, _ = vegas.candidate_uniforms(vegas.MAXF_SEED, sid, np.arange(n, dtype=np.uint64), g.ndim)
+    x, J = vegas.map_points(g, ie, y)
+    w = J * INTEGRAND[P](x, ev)
+    ref = z[f"xsec__{P}__{mv}__{T.material}"][ie]
+    assert abs(ref[0] / g.E[ie] - 1) < 1e-12
+    return w.mean(), w.std(), ref[1], len(g.E)
+
+
+@pytest.mark.parametrize("P,mv", [("DarkBrem", "0.003"), ("DarkBrem", "0.03"), ("DarkBrem", "0.3"),
+                                  ("DarkMuonBrem", "0.03"), ("DarkMuonBrem", "0.3"),
+                                  ("DarkComp", "0.003"), ("DarkComp", "0.03"), ("DarkComp", "0.3")])
+@pytest.mark.parametrize("mat", ["graphite", "lead"])
+def test_dark_sigma_table_identity(tabs, P, mv, mat):
+    T = tabs[mat]
+    n_E = _dark_moments(T, P, mv, 0, n=16)[3]
+    for ie in (5, n_E // 2, n_E - 3):
+        mean, sd, ref, _ = _dark_moments(T, P, mv, ie)
+        sig = sd * np.sqrt(1.0 / N + 1.0 / N_REF)
+        assert abs(mean - ref) <= 5 * sig + DARK_FLOOR[P] * abs(ref), (P, mv, mat, ie, mean, ref, sig)
+
+
+@pytest.mark.parametrize("mv", ["0.003", "0.03", "0.3"])
+def test_dark_ann_table_band(tabs, mv):
+    T = tabs["graphite"]
+    n_E = _dark_moments(T, "DarkAnn", mv, 0, n=16)[3]
+    for ie in (5, n_E // 2, n_E - 3):
+        mean, _, ref, _ = _dark_moments(T, "DarkAnn", mv, ie)
+        assert 0.6 * ref < mean < 1.15 * ref, (mv, ie, mean, ref)
