@@ -10,6 +10,11 @@ its descendants to the end (PETITE's generate_shower for each primary).
 Prints one JSON line: particle-steps/s (value), showers/s, the roofline of
 the dominant kernel (from HIP events on the run's stream) and a CPU baseline
 (the float64 oracle, one core, on a bounded sample of the same primaries).
+Two dark workloads follow under "dark_forward" (20,000 e- at 10 GeV in
+graphite, m_V = 1 MeV: vectors/s) and "dark_yield_scan" (a fixed record of
+50,000 e- showers in lead, yield and d/dm_V at 32 masses: gradient
+evaluations/s and the gradient/forward cost ratio), each with its CPU
+baseline.
 """
 import argparse
 import json
@@ -87,6 +92,7 @@ def main():
     ap.add_argument("--primaries", type=int, default=N_PRIMARIES)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-dark", action="store_true", help="skip the dark-vector and dark-yield workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): showers are independent, so a multi-GPU "
@@ -155,6 +161,25 @@ def main():
         maxf = {(P, ie): float(v) for P, arr in ctx.maxf.items() for ie, v in enumerate(arr)}
         cpu = cpu_baseline(prim, maxf, args.cpu_budget)
 
+    dark_out = yield_out = None
+    if not args.no_dark:
+        del sh, ctx
+        torch.cuda.empty_cache()
+        dark_out, ds, rec, dark = bench_dark_forward()
+        print(f"dark: {dark_out['vectors']} vectors from {dark_out['sm_records']} records", file=sys.stderr, flush=True)
+        if not args.no_cpu_baseline:
+            dark_out["cpu_baseline"] = cpu_dark_baseline(ds, rec)
+            dark_out["gpu_over_cpu_vectors"] = dark_out["vectors_per_s"] / dark_out["cpu_baseline"]["value"]
+        del ds, rec, dark
+        torch.cuda.empty_cache()
+        yield_out, ds, rec, model = bench_yield()
+        print(f"yield: {yield_out['sm_records']} records, grad/forward {yield_out['grad_over_forward']:.2f}",
+              file=sys.stderr, flush=True)
+        if not args.no_cpu_baseline:
+            yield_out["cpu_baseline"] = cpu_yield_baseline(rec, model)
+            yield_out["gpu_over_cpu_grad_evals"] = yield_out["grad_evals_per_s"] / yield_out["cpu_baseline"]["value"]
+        del ds, rec, model
+
     out = {
         "metric": "particle-steps/s (batched SM showers, 1 MI355X)",
         "value": steps_per_s,
@@ -181,12 +206,178 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu_showers"] = showers_per_s / cpu["showers_per_s"]
+    if dark_out:
+        out["dark_forward"] = dark_out
+    if yield_out:
+        out["dark_yield_scan"] = yield_out
     print(json.dumps(out))
 
 
 # float operations of one accept/reject candidate, averaged over the mix of
 # processes (map transform + integrand; Philox integer work excluded)
 FLOPS_PER_CANDIDATE = 120
+
+# dark workloads
+DARK_N, DARK_E, DARK_MAT, DARK_MV = 20_000, 10.0, "graphite", 0.001
+YIELD_N, YIELD_MAT = 50_000, "lead"
+YIELD_MASSES = np.geomspace(1e-3, 0.1, 32)
+# algorithmic bytes: the dark selection reads a 64-byte SM record and the
+# emission writes a 48-byte vector record; the yield pass reads pid, E0 and
+# the weight (12 bytes) of every record once per batch of 8 masses
+B_DARK_SM, B_DARK_V, B_YIELD = 64, 48, 12
+
+
+def _electrons(n, energies, seed):
+    from petite_amd.batch import PRIMARY_DTYPE
+    from petite_amd.constants import m_electron
+    E = np.asarray(energies, dtype=np.float64)
+    prim = np.zeros(n, dtype=PRIMARY_DTYPE)
+    prim["E"] = E
+    prim["pz"] = np.sqrt(E * E - m_electron ** 2)
+    prim["mass"] = m_electron
+    prim["weight"] = 1.0
+    prim["pid"] = 11
+    prim["shower"] = np.arange(n, dtype=np.int32)
+    prim["sid"] = np.arange(n, dtype=np.uint64) + (np.uint64(seed) << np.uint64(40))
+    return prim
+
+
+def bench_dark_forward(n=DARK_N, reps=2):
+    """20,000 e- at 10 GeV in graphite, m_V = 1 MeV (trained tables of 3 MeV,
+    mode='approx'), DarkBrem/DarkAnn/DarkComp: SM record, then the vectors."""
+    import torch
+    from petite_amd.dark_shower import DarkShower
+    ds = DarkShower("100GeV", DARK_MAT, CUTOFF, DARK_MV, mode="approx", seed=SEED,
+                    active_processes=["DarkBrem", "DarkAnn", "DarkComp"])
+    ctx = ds.device_context()
+    ctx.set_timing(True)
+    prim = _electrons(n, np.full(n, DARK_E), 3)
+    rec = ds.generate_showers(prim)
+    print(f"dark: SM record of {rec.n} particles", file=sys.stderr, flush=True)
+    dark = ds.generate_dark_showers(rec)  # warm-up
+    print(f"dark: warm-up reweight, {dark.n} entries", file=sys.stderr, flush=True)
+    del dark
+    torch.cuda.synchronize()
+    t_sm = t_dk = 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        rec = ds.generate_showers(prim)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        dark = ds.generate_dark_showers(rec)
+        torch.cuda.synchronize()
+        t_sm += t1 - t0
+        t_dk += time.perf_counter() - t1
+        print(f"dark: rep {t_dk:.3f} s", file=sys.stderr, flush=True)
+    st = dark.stats
+    n_sm, n_v = rec.n, st["n_vectors"]
+    k_ms = st["ms_select"] + st["ms_draw"] + st["ms_emit"]
+    alg = B_DARK_SM * n_sm + B_DARK_V * st["n_entries"]
+    out = {"workload": f"{n} e- at {DARK_E} GeV, {DARK_MAT}, cutoff {CUTOFF} GeV, m_V={DARK_MV} GeV (approx on the "
+                       f"3 MeV tables), DarkBrem/DarkAnn/DarkComp, seed {SEED}",
+           "sm_records": n_sm, "entries": st["n_entries"], "vectors": n_v,
+           "yield_per_eps2": sum(st["yield"]), "ms_sm_record": t_sm / reps * 1e3, "ms_dark": t_dk / reps * 1e3,
+           "vectors_per_s": n_v / (t_dk / reps), "sm_records_reweighted_per_s": n_sm / (t_dk / reps),
+           "showers_per_s_sm_plus_dark": n / ((t_sm + t_dk) / reps),
+           "kernels_ms": {"select": st["ms_select"], "draw": st["ms_draw"], "emit": st["ms_emit"]},
+           "trials_per_draw": st["n_trials"] / max(st["n_draws"], 1),
+           "roofline_select_emit": {"bound": "hbm", "achieved": alg / ((st["ms_select"] + st["ms_emit"]) / 1e3) / 1e9,
+                                    "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                    "per_unit": "64 B per SM record + 48 B per entry"},
+           "kernel_ms_total": k_ms}
+    return out, ds, rec, dark
+
+
+def bench_yield(reps=5):
+    """Fixed record of 50,000 e- showers (1-20 GeV log-uniform, lead), yield
+    and d/dm_V at 32 m_V in [1, 100] MeV; d/d(eps^2) is the yield itself."""
+    import torch
+    from petite_amd.dark_shower import DarkShower
+    ds = DarkShower("100GeV", YIELD_MAT, CUTOFF, 0.01, seed=SEED)
+    rng = np.random.default_rng(SEED + 1)
+    prim = _electrons(YIELD_N, np.exp(rng.uniform(0.0, math.log(20.0), YIELD_N)), 4)
+    rec = ds.generate_showers(prim)
+    print(f"yield: record of {rec.n} particles", file=sys.stderr, flush=True)
+    ds.dark_yield(rec, torch.tensor(YIELD_MASSES[:1]), 1.0)
+    model = ds._yield_model
+    model.evaluate(rec, YIELD_MASSES)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Yf, _ = model.evaluate(rec, YIELD_MASSES, grad=False)
+    t_f = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Y, dY = model.evaluate(rec, YIELD_MASSES)
+    t_g = (time.perf_counter() - t0) / reps
+    # finite differences (step 1e-3 m_V) at masses away from the trained ones
+    inner = [k for k, m in enumerate(YIELD_MASSES)
+             if all(abs(m / t - 1) > 2e-3 for t in model.masses) and model.masses[0] < m < model.masses[-1]]
+    mv = YIELD_MASSES[inner]
+    Yp, _ = model.evaluate(rec, mv * (1 + 1e-3), grad=False)
+    Ym, _ = model.evaluate(rec, mv * (1 - 1e-3), grad=False)
+    fd = (Yp - Ym) / (2e-3 * mv)
+    fd_err = float(np.max(np.abs(fd - dY[inner]) / np.maximum(np.abs(dY[inner]), 1e-300)))
+    passes = math.ceil(len(YIELD_MASSES) / 8)
+    alg = B_YIELD * rec.n * passes
+    out = {"workload": f"{YIELD_N} e- showers, E log-uniform [1,20] GeV, {YIELD_MAT}, cutoff {CUTOFF} GeV; "
+                       f"32 m_V in [1,100] MeV; eps^2 enters linearly",
+           "sm_records": rec.n, "trained_masses": model.masses,
+           "ms_forward": t_f * 1e3, "ms_forward_and_grad": t_g * 1e3, "grad_over_forward": t_g / t_f,
+           "scans_per_s": 1.0 / t_g, "grad_evals_per_s": len(YIELD_MASSES) / t_g,
+           "fd_max_rel_err": fd_err, "fd_points": len(inner),
+           "roofline": {"bound": "hbm", "achieved": alg / t_g / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "per_unit": "12 B per record per 8-mass pass (host-timed, includes launch and readback)"},
+           "yield_per_eps2_at_10MeV": float(np.interp(0.01, YIELD_MASSES, Y))}
+    return out, ds, rec, model
+
+
+def cpu_dark_baseline(ds, rec, budget_s=8.0):
+    """Oracle produce_bsm_particle over the first records of the dark workload, one core."""
+    from oracle.dark import OracleDark, OracleDarkSampler
+    from oracle.tables import SMTables
+    from petite_amd.constants import mass_dict
+    ctx = ds.device_context()
+    od = OracleDark(SMTables(DARK_MAT), DARK_MV, mode="approx", with_integrals=False)
+    s = OracleDarkSampler(od, ds._dark.W, ds._dark.D, {P: ctx.maxf[P] for P in ("DarkBrem", "DarkAnn", "DarkComp")},
+                          max_trials=ctx.desc.max_trials)
+    a = {k: rec.tensor(k)[:20000].cpu().numpy() for k in ("pid", "p0", "pf", "rf", "r0", "sid")}
+    t0 = time.perf_counter()
+    n = nv = 0
+    while n < len(a["pid"]) and time.perf_counter() - t0 < budget_s:
+        pid = int(a["pid"][n])
+        r = {"pid": pid, "mass": mass_dict[pid], "p0": a["p0"][n].astype(np.float64),
+             "pf": a["pf"][n].astype(np.float64), "rf": a["rf"][n][:3].astype(np.float64),
+             "weight": float(a["r0"][n][3]), "sid": np.uint64(a["sid"][n].view(np.uint64))}
+        for P in ("DarkBrem", "DarkAnn", "DarkComp"):
+            wg = s.weight(pid, r["p0"][0], P)
+            if wg > 0 and s.produce(r, P, wg, SEED) is not None:
+                nv += 1
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": nv / dt, "unit": "dark vectors/s", "cores": 1, "kind": "port",
+            "sm_records_per_s": n / dt,
+            "sample": f"first {n} SM records of the dark workload, {dt:.1f} s on one core (oracle float64)"}
+
+
+def cpu_yield_baseline(rec, model, budget_records=200_000):
+    """Oracle float64 yield + gradient over a bounded prefix of the record, one core."""
+    from oracle.dark import OracleDark, OracleDarkSampler, yield_and_grad
+    from oracle.tables import SMTables
+    T = SMTables(YIELD_MAT)
+    samplers = [OracleDarkSampler(OracleDark(T, m, with_integrals=False), model.dark[m].W, model.dark[m].D, None)
+                for m in model.masses]
+    n = min(rec.n, budget_records)
+    pid = rec.tensor("pid")[:n].cpu().numpy()
+    E0 = rec.tensor("p0")[:n, 0].double().cpu().numpy()
+    w = rec.tensor("r0")[:n, 3].double().cpu().numpy()
+    procs = ["DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"]
+    t0 = time.perf_counter()
+    yield_and_grad(samplers, model.masses, procs, pid, E0, w, YIELD_MASSES)
+    dt = time.perf_counter() - t0
+    scans = (n / rec.n) / dt
+    return {"value": scans * len(YIELD_MASSES), "unit": "grad evals/s (full record)", "cores": 1, "kind": "port",
+            "sample": f"first {n} of {rec.n} records, 32 masses, {dt:.2f} s (numpy float64), scaled to the record"}
 
 if __name__ == "__main__":
     main()

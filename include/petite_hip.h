@@ -244,7 +244,8 @@ int petite_yield_setup(PetiteCtx* ctx, int32_t n_mass, const double* masses, con
                        const double* blob, int64_t n);
 
 /* Y and dY/dm_V at n_points masses (host arrays), accumulated in float64 over
- * the first n_sm records of the store. */
+ * the first n_sm records of the store.  dyield_dmV_host may be NULL: the
+ * yield alone, without the derivative work. */
 int petite_dark_yield(PetiteCtx* ctx, const PetiteStore* sm, int64_t n_sm, int32_t n_points, const double* mV_host,
                       double* yield_host, double* dyield_dmV_host, PetiteStream stream);
 

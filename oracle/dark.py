@@ -385,6 +385,11 @@ class OracleDarkSampler:
             V = np.array([math.sqrt(E ** 2 - m_electron ** 2 + mV ** 2), 0.0, 0.0, math.sqrt(E ** 2 - m_electron ** 2)])
         elif process == "DarkComp" and E <= od.E_C:
             V = np.array([math.sqrt(E ** 2 + mV ** 2), 0.0, 0.0, E])
+        elif process == "DarkAnn" and not mV > 2 * m_electron:
+            # the radiative-return integrand carries sqrt(1 - 4 m_e^2/m_V^2)
+            # (all_processes.py:431): NaN at every point, so draw_dark_sample's
+            # loop ends without a sample
+            return None
         else:
             grid = od._grids[process]
             ie = grid.lu_key(E)

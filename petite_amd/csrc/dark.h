@@ -428,7 +428,9 @@ struct YieldPoints {
   double dtdm[kYieldBatch];
 };
 
-// One float64 pass over the record: acc[2k] += w W(m_k), acc[2k+1] += w dW/dm(m_k).
+// One float64 pass over the record: acc[2k] += w W(m_k), acc[2k+1] += w dW/dm(m_k)
+// (GRAD = false: the yield alone).
+template <bool GRAD>
 __global__ __launch_bounds__(256) void k_dark_yield(DevTables T, YieldTables Y, DevStore S, int64_t n, YieldPoints P,
                                                     double* acc) {
   double y[kYieldBatch], dy[kYieldBatch];
@@ -463,13 +465,13 @@ __global__ __launch_bounds__(256) void k_dark_yield(DevTables T, YieldTables Y, 
         if (lo > 0.0 && hi > 0.0) {
           const double r = log(hi / lo);
           W = lo * exp(t * r);
-          dW = W * r * P.dtdm[k];
+          dW = GRAD ? W * r * P.dtdm[k] : 0.0;
         } else {
           W = lo + t * (hi - lo);
-          dW = (hi - lo) * P.dtdm[k];
+          dW = GRAD ? (hi - lo) * P.dtdm[k] : 0.0;
         }
         y[k] += w * W;
-        dy[k] += w * dW;
+        if (GRAD) dy[k] += w * dW;
       }
     }
   }
@@ -478,11 +480,11 @@ __global__ __launch_bounds__(256) void k_dark_yield(DevTables T, YieldTables Y, 
     double a = y[k], b = dy[k];
     for (int off = 32; off > 0; off >>= 1) {
       a += shfl_f64(a, (threadIdx.x & 63) ^ off);
-      b += shfl_f64(b, (threadIdx.x & 63) ^ off);
+      if (GRAD) b += shfl_f64(b, (threadIdx.x & 63) ^ off);
     }
     if ((threadIdx.x & 63) == 0 && k < P.n) {
       unsafeAtomicAdd(acc + 2 * k, a);
-      unsafeAtomicAdd(acc + 2 * k + 1, b);
+      if (GRAD) unsafeAtomicAdd(acc + 2 * k + 1, b);
     }
   }
 }

@@ -140,6 +140,15 @@ constexpr int proc_ndim() {
   return (P == BREM || P == MUONBREM || P == PAIRPROD) ? 4 : ((P == DARKBREM || P == DARKMUONBREM) ? 3 : 1);
 }
 
+// False when the integrand is NaN at every point, so that draw_sample's loop
+// can only end without a sample: the radiative-return width factor
+// sqrt(1 - 4 m_e^2 / m_V^2) (all_processes.py:431) for m_V <= 2 m_e.
+template <int P>
+__device__ inline bool draw_possible(const DevTables& T) {
+  if (P == DARKANN) return T.d->mV > 2.0 * kMe;
+  return true;
+}
+
 template <int P, int ND>
 __device__ inline double candidate_w(const DevTables& T, double E, const float* base, const int32_t* ninc,
                                      uint64_t seed, uint64_t sid, uint32_t j, double* x, double& ua) {
@@ -198,7 +207,9 @@ __device__ void draw_loop(const DevTables& T, const DrawLaunch& io, uint64_t see
           const int ie = lu_key(T, p, E);
           base = T.f32 + gd.node_off + (int64_t)ie * gd.stride;
           mf = T.maxf[p * T.maxE + ie] * T.d->maxF_fudge;
-          j = 0;
+          // a draw that cannot accept ends after its first round, as the
+          // reference's loop would after max_trials candidates
+          j = draw_possible<p>(T) ? 0u : max_trials;
           have = true;
         } else {
           done = true;
@@ -1352,7 +1363,7 @@ int petite_yield_setup(PetiteCtx* c, int32_t n_mass, const double* masses, const
 
 int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t n_points, const double* mV,
                       double* yield_host, double* dyield_host, PetiteStream stream) {
-  if (!c || !sm || n_sm < 0 || n_points < 0 || (n_points > 0 && (!mV || !yield_host || !dyield_host)))
+  if (!c || !sm || n_sm < 0 || n_points < 0 || (n_points > 0 && (!mV || !yield_host)))
     return fail(PETITE_ERR_ARG, "petite_dark_yield: bad arguments");
   if (c->n_mass < 1) return fail(PETITE_ERR_ARG, "petite_dark_yield: no trained masses (petite_yield_setup)");
   if (n_sm > sm->capacity) return fail(PETITE_ERR_ARG, "petite_dark_yield: n_sm above the store capacity");
@@ -1390,8 +1401,12 @@ int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t
     }
     HIP_TRY(hipMemsetAsync(c->yacc, 0, 2 * dark::kYieldBatch * sizeof(double), s));
     if (n_sm > 0) {
-      hipLaunchKernelGGL(dark::k_dark_yield, dim3((unsigned)blocks), dim3(256), 0, s, c->dev, c->ytab, S, n_sm, P,
-                         c->yacc);
+      if (dyield_host)
+        hipLaunchKernelGGL(dark::k_dark_yield<true>, dim3((unsigned)blocks), dim3(256), 0, s, c->dev, c->ytab, S, n_sm,
+                           P, c->yacc);
+      else
+        hipLaunchKernelGGL(dark::k_dark_yield<false>, dim3((unsigned)blocks), dim3(256), 0, s, c->dev, c->ytab, S,
+                           n_sm, P, c->yacc);
       HIP_TRY(hipGetLastError());
     }
     double h[2 * dark::kYieldBatch];
@@ -1399,7 +1414,7 @@ int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t
     HIP_TRY(hipStreamSynchronize(s));
     for (int k = 0; k < P.n; ++k) {
       yield_host[k0 + k] = h[2 * k];
-      dyield_host[k0 + k] = h[2 * k + 1];
+      if (dyield_host) dyield_host[k0 + k] = h[2 * k + 1];
     }
   }
   return PETITE_OK;

@@ -65,16 +65,17 @@ class DarkYield:
                                                    C.cast(descs, C.c_void_p), self._blob.ctypes.data,
                                                    self._blob.size), "petite_yield_setup")
 
-    def evaluate(self, record, mV):
-        """(Y, dY/dm_V) per unit eps^2 at the masses mV (host float64 arrays)."""
+    def evaluate(self, record, mV, grad=True):
+        """(Y, dY/dm_V) per unit eps^2 at the masses mV (host float64 arrays);
+        grad=False skips the derivative (returns None for it)."""
         store, n = (record._store, record.n) if hasattr(record, "_store") else record
         mv = np.ascontiguousarray(np.atleast_1d(mV), dtype=np.float64)
         Y = np.zeros_like(mv)
-        dY = np.zeros_like(mv)
+        dY = np.zeros_like(mv) if grad else None
         S = _store_struct(store)
         _lib.check(self.ctx.lib.petite_dark_yield(self.ctx.ctx, C.byref(S), int(n), mv.size, mv.ctypes.data,
-                                                  Y.ctypes.data, dY.ctypes.data, _stream_handle()),
-                   "petite_dark_yield")
+                                                  Y.ctypes.data, dY.ctypes.data if grad else None,
+                                                  _stream_handle()), "petite_dark_yield")
         return Y, dY
 
     def __call__(self, record, mV, eps2):

@@ -137,3 +137,21 @@ def test_drop_in_lists(run):
         assert d["generation_process"] in ("DarkBrem", "DarkAnn_bound", "DarkComp_bound", "DarkMuonBrem")
     sm2, vecs2 = ds.generate_dark_shower(ExDir=sm)
     assert sm2 is sm and len(vecs2) == len(vecs)
+
+
+def test_light_vector_radiative_return_has_no_sample():
+    # m_V = 1 MeV < 2 m_e: the radiative-return integrand is NaN everywhere, so
+    # every DarkAnn draw ends without a sample (only the threshold shortcut of
+    # positrons below E_res gives a vector); the call returns promptly
+    ds = DarkShower("100GeV", "graphite", 0.010, 0.001, mode="approx", seed=2,
+                    active_processes=["DarkBrem", "DarkAnn"])
+    prims = [_prim(E, -11) for E in (0.5, 1.0, 3.0)]
+    batch = ds.generate_showers(prims)
+    dark = ds.generate_dark_showers(batch)
+    proc = dark.process.cpu().numpy()
+    val = dark.valid.cpu().numpy()
+    E = dark.p.cpu().numpy()[:, 0]
+    ann = proc == 9
+    assert ann.sum() > 0
+    assert dark.stats["n_no_sample"] == int((ann & ~val).sum()) > 0
+    assert np.all(val[proc == 8])
