@@ -119,9 +119,13 @@ class SMTables:
     CHANNELS = {11: ("Brem", "Moller"), -11: ("Brem", "Ann", "Bhabha"), 22: ("PairProd", "Comp"),
                 13: ("MuonE", "MuonBrem"), -13: ("MuonE", "MuonBrem")}
 
+    # summation order of _NSigmaElectron/Photon/Positron/Muon (shower.py:355-366)
+    SUM_ORDER = {11: ("Brem", "Moller"), -11: ("Brem", "Bhabha", "Ann"), 22: ("PairProd", "Comp"),
+                 13: ("MuonBrem", "MuonE"), -13: ("MuonBrem", "MuonE")}
+
     def nsigma_total(self, pid, E):
         # _NSigmaElectron/Photon/Positron/Muon, shower.py:357-368
-        return sum(self.nsigma(p, E) for p in self.CHANNELS[pid])
+        return sum(self.nsigma(p, E) for p in self.SUM_ORDER[pid])
 
     def mfp(self, pid, E):
         """get_mfp, shower.py:370-389: metres; 1e12 when n*sigma <= 0."""

@@ -407,10 +407,13 @@ struct Requests {
 };
 
 // n*sigma tables in LDS.  Header per process p (4 doubles): ln E0,
-// (n-1)/ln(E_last/E0), offset of the energies, n; values follow the energies.
+// (n-1)/ln(E_last/E0), offset of the energies, n; the energies, the values and
+// the interval slopes (y[i+1]-y[i])/(x[i+1]-x[i]) follow, so a lookup is
+// slope * (E - x[lo]) + y[lo], the arithmetic of scipy's interp1d.
 struct LdsSigma {
   const double* base;
-  __device__ inline double nsigma(int p, double E) const {
+  // lf = logf(E), shared by the lookups of one energy (geometric-grid guess)
+  __device__ inline double nsigma(int p, double E, float lf) const {
     const double* h = base + 4 * p;
     const int n = (int)h[3];
     if (n <= 0) return 0.0;
@@ -418,15 +421,31 @@ struct LdsSigma {
     const double* ye = xe + n;
     if (!(E >= xe[0]) || E > xe[n - 1]) return 0.0;
     if (n == 1) return ye[0];
-    const int lo = grid_interval(xe, n, h[0], h[1], E);
-    const double t = (E - xe[lo]) / (xe[lo + 1] - xe[lo]);
-    return ye[lo] + t * (ye[lo + 1] - ye[lo]);
+    int lo;
+    const double inv_dlog = h[1];
+    if (inv_dlog > 0.0) {
+      lo = (int)((lf - (float)h[0]) * (float)inv_dlog);
+      lo = lo < 0 ? 0 : (lo > n - 2 ? n - 2 : lo);
+      while (lo > 0 && xe[lo] > E) --lo;
+      while (lo < n - 2 && xe[lo + 1] <= E) ++lo;
+    } else {
+      lo = grid_interval(xe, n, 0.0, 0.0, E);
+    }
+    return ye[n + lo] * (E - xe[lo]) + ye[lo];
+  }
+  // _NSigmaElectron/Photon/Positron/Muon (shower.py:355-366), terms in their order
+  __device__ inline double nsigma_sum(int pid, double E) const {
+    const float lf = __logf((float)E);
+    switch (pid) {
+      case 11: return nsigma(BREM, E, lf) + nsigma(MOLLER, E, lf);
+      case -11: return nsigma(BREM, E, lf) + nsigma(BHABHA, E, lf) + nsigma(ANN, E, lf);
+      case 22: return nsigma(PAIRPROD, E, lf) + nsigma(COMP, E, lf);
+      case 13: case -13: return nsigma(MUONBREM, E, lf) + nsigma(MUONE, E, lf);
+      default: return 0.0;
+    }
   }
   __device__ inline double mfp(int pid, double E) const {
-    int ch[3];
-    const int n = channels(pid, ch);
-    double s = 0.0;
-    for (int i = 0; i < n; ++i) s += nsigma(ch[i], E);
+    const double s = nsigma_sum(pid, E);
     return s > 0.0 ? kCmToM / s : 1.0e12;
   }
 };
@@ -478,10 +497,13 @@ __device__ inline void charged_advance(const DevTables& T, double dist, double m
 
 // ---------------------------------------------------------- transport -----
 // propagate_particle (shower.py:509-601) and the process choice (665-698)
-// for the store slots [first, first + n).  One substep per loop iteration;
-// lanes whose particle ended take new particles once a quarter of the wave
-// is idle (or all of it), so uneven substep counts do not idle the wave and
-// the record loads of new particles are paid once per batch.
+// for the store slots [first, first + n).  One advance per loop iteration.
+// Refills (record loads, photon flights) and finishes (record stores, process
+// choice, draw requests) run for the whole wave at once, so they wait until
+// kBatchLanes lanes need them (or no lane is stepping): each costs about as
+// much as several substeps, and measured on MI355X 56 of 64 lanes gives the
+// shortest sweeps (16: 1.7x longer).
+constexpr int kBatchLanes = 56;
 __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int64_t first, int64_t n, uint64_t seed,
                                                    Requests R, double* shower_energy, unsigned long long* counters,
                                                    const double* sig_lds_src) {
@@ -496,70 +518,17 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
   int64_t slot = 0;
   int pid = 0, shower = 0;
   uint64_t sid = 0;
-  double m = 0.0, pme = 0.0, E = 0.0, p = 0.0, dep = 0.0;
+  double m = 0.0, pme = 0.0, E = 0.0, p = 0.0, dep = 0.0, dz = 0.0;
   float nx = 0.0f, ny = 0.0f, nz = 0.0f, rx = 0.0f, ry = 0.0f, rz = 0.0f;
   uint32_t k = 0;
+  bool closing = false;  // the next advance is the closing increment
   unsigned long long steps = 0;
   while (true) {
     const unsigned long long am = __ballot(act);
-    // ---- finish parked particles in batches: write pf/rf, choose the process, queue the draw
-    const unsigned long long wm = __ballot(wait);
-    if (wm && (__popcll(wm) >= 16 || am == 0ull)) {
-      int want = -1;
-      if (wait) {
-        steps += k;
-        S.pf[slot] = make_float4((float)E, (float)(p * nx), (float)(p * ny), (float)(p * nz));
-        S.rf[slot] = make_float4(rx, ry, rz, (float)k);
-        // process choice (shower.py:665-698) behind sample_scattering's threshold (469-470)
-        if (E > pme) {
-          int ch[3];
-          const int nch = channels(pid, ch);
-          double wv[3], SC = 0.0;
-          for (int c = 0; c < nch; ++c) {
-            wv[c] = L.nsigma(ch[c], E);
-            SC += wv[c];
-          }
-          if (nch > 0 && SC > 0.0 && !isnan(SC)) {
-            const double uu = u01d(draw_block(seed, sid, 0, PURPOSE_INTERACT).x) * SC;
-            double c = 0.0;
-            int pick = nch - 1;
-            for (int qq = 0; qq < nch; ++qq) {
-              c += wv[qq];
-              if (uu < c) {
-                pick = qq;
-                break;
-              }
-            }
-            want = ch[pick];
-          }
-        }
-        if (shower_energy) {
-          unsafeAtomicAdd(shower_energy + 4 * shower + 0, dep);
-          if (want < 0) unsafeAtomicAdd(shower_energy + 4 * shower + 1, E);
-        }
-        wait = false;
-      }
-      // wave-aggregated appends to the per-process request regions (SM processes)
-      if (__ballot(want >= 0)) {
-        for (int pp = 0; pp < 8; ++pp) {
-          const unsigned long long mask = __ballot(want == pp);
-          if (!mask) continue;
-          const int leader = __ffsll((long long)mask) - 1;
-          unsigned base = 0;
-          if (lane == leader) base = atomicAdd(R.count + pp, (unsigned)__popcll(mask));
-          base = __shfl(base, leader);
-          if (want == pp) {
-            const unsigned idx = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
-            R.slot[(int64_t)pp * R.cap + idx] = (int32_t)slot;
-            R.E[(int64_t)pp * R.cap + idx] = E;
-          }
-        }
-      }
-    }
     // ---- refill empty lanes in batches
     const bool idle = !act && !wait && !done;
     const unsigned long long im = __ballot(idle);
-    if (im && (__popcll(im) >= 16 || am == 0ull)) {
+    if (im && (__popcll(im) >= kBatchLanes || am == 0ull)) {
       const int64_t r = q.take(im, idle, counters + C_TRANSPORT);
       if (idle) {
         if (r < n) {
@@ -588,6 +557,7 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
           rz = r0.z;
           dep = 0.0;
           k = 0;
+          closing = false;
           if (E < pme) {
             wait = true;  // ends where it starts (shower.py:532-536)
           } else if (pid == 22) {
@@ -607,34 +577,114 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
         }
       }
     }
+    // ---- finish parked particles in batches: write pf/rf, choose the process, queue the draw
+    const unsigned long long wm = __ballot(wait);
+    if (wm && (__popcll(wm) >= kBatchLanes || __ballot(act) == 0ull)) {
+      int want = -1;
+      if (wait) {
+        steps += k;
+        S.pf[slot] = make_float4((float)E, (float)(p * nx), (float)(p * ny), (float)(p * nz));
+        S.rf[slot] = make_float4(rx, ry, rz, (float)k);
+        // process choice (shower.py:665-698) behind sample_scattering's threshold (469-470)
+        if (E > pme) {
+          // channels in the order of shower.py:665-698, held in registers
+          int c0 = -1, c1 = -1, c2 = -1;
+          switch (pid) {
+            case 11: c0 = BREM; c1 = MOLLER; break;
+            case -11: c0 = BREM; c1 = ANN; c2 = BHABHA; break;
+            case 22: c0 = PAIRPROD; c1 = COMP; break;
+            case 13: case -13: c0 = MUONE; c1 = MUONBREM; break;
+            default: break;
+          }
+          if (c0 >= 0) {
+            const float lf = __logf((float)E);
+            const double w0 = L.nsigma(c0, E, lf), w1 = L.nsigma(c1, E, lf);
+            const double w2 = c2 >= 0 ? L.nsigma(c2, E, lf) : 0.0;
+            const double SC = c2 >= 0 ? w0 + w1 + w2 : w0 + w1;
+            if (SC > 0.0 && !isnan(SC)) {
+              const double uu = u01d(draw_block(seed, sid, 0, PURPOSE_INTERACT).x) * SC;
+              if (uu < w0) want = c0;
+              else if (c2 < 0 || uu < w0 + w1) want = c1;
+              else want = c2;
+            }
+          }
+        }
+        if (shower_energy) {
+          unsafeAtomicAdd(shower_energy + 4 * shower + 0, dep);
+          if (want < 0) unsafeAtomicAdd(shower_energy + 4 * shower + 1, E);
+        }
+        wait = false;
+      }
+      // wave-aggregated appends to the per-process request regions (SM processes)
+      if (__ballot(want >= 0)) {
+        for (int pp = 0; pp < 8; ++pp) {
+          const unsigned long long mask = __ballot(want == pp);
+          if (!mask) continue;
+          const int leader = __ffsll((long long)mask) - 1;
+          unsigned base = 0;
+          if (lane == leader) base = atomicAdd(R.count + pp, (unsigned)__popcll(mask));
+          base = __shfl(base, leader);
+          if (want == pp) {
+            const unsigned idx = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+            R.slot[(int64_t)pp * R.cap + idx] = (int32_t)slot;
+            R.E[(int64_t)pp * R.cap + idx] = E;
+          }
+        }
+      }
+    }
     if (__ballot(act) == 0ull) {
       if (__ballot(wait || !done) == 0ull) break;
       continue;
     }
     if (act) {
-      // one substep (shower.py:559-581); the hard test uses the substep's start mfp.
-      // exp in float32: the comparison flips only when u0 lies within ~1e-7 of the threshold
-      const double mfp = L.mfp(pid, E);
-      const U4 u = draw_block(seed, sid, k, PURPOSE_STEP);
-      const double dz = mfp / (6.0 + 14.0 * u01d(u.y));
-      const bool hard = u01d(u.x) > (double)expf(-(float)(dz / mfp));
-      if (!hard) charged_advance(T, dz, m, m, u.z, u.w, E, p, nx, ny, nz, rx, ry, rz, dep);
-      ++k;
-      if (hard || E < pme) {
-        // closing increment (shower.py:583-598); the reference omits m_lepton in its MCS call
-        const U4 c = draw_block(seed, sid, 0, PURPOSE_CLOSE);
-        const double distC = u01d(c.x);
-        double dist;
+      // One advance per lane and iteration, so the lanes of a wave run the same
+      // code: a substep (shower.py:559-581), or the closing increment
+      // (583-598) in the iteration after the substep that ended the loop.
+      //   substep: dz = mfp / U(6,20); hard if u > exp(-dz/mfp) = exp(-1/U(6,20)),
+      //            else lose dE/dx dz, move, scatter.  exp in float32: the
+      //            comparison flips only when u lies within ~1e-7 of the threshold.
+      //   closing: below the cutoff dist = u dz, else the truncated exponential
+      //            dist = -mfp ln(1 + (exp(-dz/mfp) - 1) u) at the current energy;
+      //            its scattering uses m_e (the reference omits m_lepton there).
+      const bool in_close = closing;
+      const double S = L.nsigma_sum(pid, E);
+      const double mfp = S > 0.0 ? kCmToM / S : 1.0e12;
+      const U4 u = draw_block(seed, sid, in_close ? 0u : k, in_close ? PURPOSE_CLOSE : PURPOSE_STEP);
+      bool adv = true;
+      double dist;
+      uint32_t w_r, w_a;
+      double m_lep = m;
+      if (!in_close) {
+        const double div = 6.0 + 14.0 * u01d(u.y);
+        dz = mfp / div;
+        dist = dz;
+        w_r = u.z;
+        w_a = u.w;
+        if (u01d(u.x) > (double)__expf(-1.0f / (float)div)) {
+          adv = false;
+          closing = true;
+        }
+      } else {
+        const double distC = u01d(u.x);
         if (E < pme) {
           dist = distC * dz;
         } else {
-          const double mfp2 = L.mfp(pid, E);
-          dist = -mfp2 * (double)log1pf(expm1f(-(float)(dz / mfp2)) * (float)distC);
+          // dz / mfp = dz * S / 0.01
+          const float r2 = S > 0.0 ? (float)(dz * S * (1.0 / kCmToM)) : (float)(dz * 1.0e-12);
+          dist = -mfp * (double)log1pf(expm1f(-r2) * (float)distC);
         }
-        charged_advance(T, dist, m, kMe, c.y, c.z, E, p, nx, ny, nz, rx, ry, rz, dep);
-        ++k;
+        w_r = u.y;
+        w_a = u.z;
+        m_lep = kMe;
+      }
+      if (adv) charged_advance(T, dist, m, m_lep, w_r, w_a, E, p, nx, ny, nz, rx, ry, rz, dep);
+      ++k;
+      if (in_close) {
+        closing = false;
         act = false;
         wait = true;
+      } else if (E < pme) {
+        closing = true;  // below the cutoff after this substep: close next
       }
     }
   }
@@ -961,6 +1011,9 @@ int petite_ctx_create(const PetiteTableDesc* desc, const double* blob64, int64_t
       lds[4 * p + 2] = (double)lds.size();
       for (int i = 0; i < sd.n; ++i) lds.push_back(blob64[sd.E_off + i]);
       for (int i = 0; i < sd.n; ++i) lds.push_back(blob64[sd.v_off + i]);
+      for (int i = 0; i + 1 < sd.n; ++i)
+        lds.push_back((blob64[sd.v_off + i + 1] - blob64[sd.v_off + i]) /
+                      (blob64[sd.E_off + i + 1] - blob64[sd.E_off + i]));
     }
     T.lds_doubles = (int32_t)lds.size();
     if (hipMalloc(&c->dsig_lds, lds.size() * sizeof(double)) != hipSuccess ||

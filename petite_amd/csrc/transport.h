@@ -151,8 +151,9 @@ __device__ inline Frame frame_of(float nx, float ny, float nz) {
   f.c_t = nz;
   f.s_t = rho;
   if (rho > 0.0f) {
-    f.c_p = nx / rho;
-    f.s_p = ny / rho;
+    const float ir = __builtin_amdgcn_rcpf(rho);
+    f.c_p = nx * ir;
+    f.s_p = ny * ir;
   } else {
     f.c_p = 1.0f;
     f.s_p = 0.0f;
@@ -184,18 +185,20 @@ __device__ inline void frame_apply_d(double nx, double ny, double nz, const V4& 
 // constants folded: chi_c^2 = cA t / (p beta)^2 and
 // v = chi_c^2 / (2 (1-F) chi_a^2) = cA t / (2 (1-F) cB (beta^2 + cC)).
 __device__ inline float lynch_dahl_theta0(float cA, float cB, float cC, float t_gcm2, float pMeV, float beta) {
+  // quotients with the hardware reciprocal (about 1 ulp): the angle is a float32 quantity
   const float F = 0.98f;
   const float pb = pMeV * beta;
-  const float chic2 = cA * t_gcm2 / (pb * pb);
-  const float v = cA * t_gcm2 / (2.0f * (1.0f - F) * cB * (beta * beta + cC));
+  const float ct = cA * t_gcm2;
+  const float chic2 = ct * __builtin_amdgcn_rcpf(pb * pb);
+  const float v = ct * __builtin_amdgcn_rcpf(2.0f * (1.0f - F) * cB * (beta * beta + cC));
   float g;  // (1+v) ln(1+v)/v - 1
   if (v < 0.1f) {
     // sum_{n>=1} (-1)^{n+1} v^n / (n (n+1))
     g = v * (0.5f + v * (-1.0f / 6.0f + v * (1.0f / 12.0f + v * (-1.0f / 20.0f + v * (1.0f / 30.0f + v * (-1.0f / 42.0f + v * (1.0f / 56.0f)))))));
   } else {
-    g = (1.0f + v) * log1pf(v) / v - 1.0f;
+    g = (1.0f + v) * log1pf(v) * __builtin_amdgcn_rcpf(v) - 1.0f;
   }
-  return sqrtf(chic2 * g / (1.0f + F * F));
+  return sqrtf(chic2 * g * (1.0f / (1.0f + F * F)));
 }
 
 }  // namespace petite
