@@ -193,9 +193,11 @@ __device__ void draw_loop(const DevTables& T, const DrawLaunch& io, uint64_t see
   int64_t oidx = 0;
   unsigned long long n_trials = 0, n_viol = 0, n_fail = 0, n_draw = 0;
   while (true) {
+    // refill once half of the groups are free (or none has a request): the
+    // refill's loads and table searches cost a whole-wave pass each time
     const bool need = (gl == 0) && !have && !done;
     const unsigned long long nm = __ballot(need);
-    if (nm) {
+    if (nm && (__popcll(nm) >= (32 >> lgG) || __ballot(have) == 0ull)) {
       const int64_t r = q.take(nm, need, io.counters + C_REFILL + p);
       if (need) {
         if (r < count) {
@@ -830,6 +832,11 @@ struct PetiteCtx {
   unsigned long long* counters = nullptr;  // C_N entries, layout CounterIdx
   bool timing = false;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  // one stream per process for the draw kernels (default priority), forked
+  // from and joined back into the caller's stream
+  hipStream_t dstream[PETITE_NPROC] = {};
+  hipEvent_t dfork = nullptr;
+  hipEvent_t djoin[PETITE_NPROC] = {};
   int n_cu = 256;
   double tbar[PETITE_NPROC];  // running mean of candidates per draw, per process
   // dark sector
@@ -927,13 +934,35 @@ static void plan_draw(PetiteCtx* c, const unsigned int* rq, DrawLaunch& io, int6
   blocks_total = io.block_start[PETITE_NPROC];
 }
 
+// The per-process kernels are independent: with more than one, each runs on
+// its own stream so small processes overlap the large ones.
 static int launch_draws(PetiteCtx* c, const DrawLaunch& io, uint64_t seed, hipStream_t s) {
+  int active[PETITE_NPROC], na = 0;
   for (int p = 0; p < PETITE_NPROC; ++p) {
     const int b = io.block_start[p + 1] - io.block_start[p];
-    if (b <= 0 || io.count[p] <= 0 || c->desc.grid[p].n_E <= 0) continue;
-    hipLaunchKernelGGL(kDrawKernels[p], dim3((unsigned)b), dim3(256), 0, s, c->dev, io, seed);
-    HIP_TRY(hipGetLastError());
+    if (b > 0 && io.count[p] > 0 && c->desc.grid[p].n_E > 0) active[na++] = p;
   }
+  if (na > 1 && !c->dfork) {
+    HIP_TRY(hipEventCreateWithFlags(&c->dfork, hipEventDisableTiming));
+    for (int p = 0; p < PETITE_NPROC; ++p) {
+      HIP_TRY(hipStreamCreateWithFlags(&c->dstream[p], hipStreamNonBlocking));
+      HIP_TRY(hipEventCreateWithFlags(&c->djoin[p], hipEventDisableTiming));
+    }
+  }
+  if (na > 1) HIP_TRY(hipEventRecord(c->dfork, s));
+  for (int i = 0; i < na; ++i) {
+    const int p = active[i];
+    const int b = io.block_start[p + 1] - io.block_start[p];
+    hipStream_t sp = s;
+    if (na > 1) {
+      sp = c->dstream[p];
+      HIP_TRY(hipStreamWaitEvent(sp, c->dfork, 0));
+    }
+    hipLaunchKernelGGL(kDrawKernels[p], dim3((unsigned)b), dim3(256), 0, sp, c->dev, io, seed);
+    HIP_TRY(hipGetLastError());
+    if (na > 1) HIP_TRY(hipEventRecord(c->djoin[p], sp));
+  }
+  for (int i = 0; na > 1 && i < na; ++i) HIP_TRY(hipStreamWaitEvent(s, c->djoin[active[i]], 0));
   return PETITE_OK;
 }
 
@@ -1033,6 +1062,11 @@ void petite_ctx_destroy(PetiteCtx* c) {
   (void)hipDeviceSynchronize();
   for (hipEvent_t& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->dfork) (void)hipEventDestroy(c->dfork);
+  for (int p = 0; p < PETITE_NPROC; ++p) {
+    if (c->djoin[p]) (void)hipEventDestroy(c->djoin[p]);
+    if (c->dstream[p]) (void)hipStreamDestroy(c->dstream[p]);
+  }
   for (void* p : {(void*)c->d64, (void*)c->d32, (void*)c->dmaxf, (void*)c->req_slot, (void*)c->req_E, (void*)c->res_x,
                   (void*)c->res_trials, (void*)c->res_w, (void*)c->req_count, (void*)c->counters, (void*)c->ddesc,
                   (void*)c->dsig_lds, (void*)c->dark_blob, (void*)c->dyield, (void*)c->yield_blob, (void*)c->yacc})
