@@ -43,6 +43,7 @@ enum CounterIdx {
   C_REFILL = 0,               // [PETITE_NPROC] per-process draw refill counters
   C_TRANSPORT = 16,           // transport refill counter
   C_STORE = 17,               // particles written to the store
+  C_SWEEPS = 18,              // sweeps with work (device-driven showers)
   C_DARK_N = 20,              // dark entries written in the current pass
   C_DARK_STATS = 21,          // [3] drate rows summing to zero, vectors written, bound-electron momentum failures
   C_STATS = 32,               // [ST_N] run statistics
@@ -68,7 +69,28 @@ struct DrawLaunch {
   int32_t* trials_out;
   float* w_out;
   unsigned long long* counters;
+  const unsigned int* dcount;  // non-null: counts on the device, plan made by each block
+  int32_t full_blocks;         // blocks of a whole-GPU launch (device plan)
 };
+
+// Launch plan of one process's draw kernel: one lane per request while there
+// are at least four requests per lane; below that, groups of 2^lg lanes share
+// a request, sized so the GPU stays filled and each group sees a few requests.
+__host__ __device__ inline void draw_plan(double cnt, int full_blocks, int& lg, int& blocks) {
+  const double full_lanes = 256.0 * full_blocks;
+  lg = 0;
+  blocks = 0;
+  if (cnt <= 0.0) return;
+  if (cnt >= 4.0 * full_lanes) {
+    blocks = full_blocks;
+    return;
+  }
+  const double want = 4.0 * full_lanes / cnt;
+  while ((1 << lg) < want && lg < 6) ++lg;
+  const double lanes = cnt * (double)(1 << lg) < full_lanes ? cnt * (double)(1 << lg) : full_lanes;
+  blocks = (int)ceil(lanes / 256.0);
+  if (blocks < 1) blocks = 1;
+}
 
 __device__ inline unsigned long long shfl_u64(unsigned long long v, int src) {
   const unsigned lo = __shfl((unsigned)(v & 0xFFFFFFFFull), src);
@@ -171,17 +193,15 @@ __device__ inline double candidate_w(const DevTables& T, double E, const float* 
 // and refills it as soon as it accepts; larger G cuts the latency of sweeps
 // with fewer requests than lanes.
 template <int p>
-__device__ void draw_loop(const DevTables& T, const DrawLaunch& io, uint64_t seed) {
+__device__ void draw_loop(const DevTables& T, const DrawLaunch& io, uint64_t seed, const int64_t count, const int lgG) {
   constexpr int ND = proc_ndim<p>();
   const int lane = threadIdx.x & 63;
-  const int lgG = io.lgG[p];
   const int G = 1 << lgG;
   const int gl = lane & (G - 1);
   const int g0 = lane - gl;
   const unsigned long long gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << g0);
   const PetiteGridDesc& gd = T.d->grid[p];
   const int32_t ninc[4] = {gd.ninc[0], gd.ninc[1], gd.ninc[2], gd.ninc[3]};
-  const int64_t count = io.count[p];
   const uint32_t max_trials = (uint32_t)(T.d->max_trials < (int64_t)0x7FFFFFF0 ? (T.d->max_trials > 0 ? T.d->max_trials : 1)
                                                                                : (int64_t)0x7FFFFFF0);
   WaveQueue<64> q;
@@ -280,8 +300,16 @@ __device__ void draw_loop(const DevTables& T, const DrawLaunch& io, uint64_t see
 // blocks [block_start[p], block_start[p+1]) of the plan, renumbered from 0.
 template <int P>
 __global__ __launch_bounds__(256) void k_draw(DevTables T, DrawLaunch io, uint64_t seed) {
-  if (io.count[P] <= 0) return;
-  draw_loop<P>(T, io, seed);
+  if (io.dcount) {
+    const int64_t cnt = io.dcount[P];
+    int lg, blocks;
+    draw_plan((double)cnt, io.full_blocks, lg, blocks);
+    if ((int)blockIdx.x >= blocks) return;
+    draw_loop<P>(T, io, seed, cnt, lg);
+  } else {
+    if (io.count[P] <= 0) return;
+    draw_loop<P>(T, io, seed, io.count[P], io.lgG[P]);
+  }
 }
 
 typedef void (*DrawKernel)(DevTables, DrawLaunch, uint64_t);
@@ -506,9 +534,16 @@ __device__ inline void charged_advance(const DevTables& T, double dist, double m
 // much as several substeps, and measured on MI355X 56 of 64 lanes gives the
 // shortest sweeps (16: 1.7x longer).
 constexpr int kBatchLanes = 56;
+// range (non-null): the sweep's first slot and count, set on the device by
+// k_sweep_setup; blocks beyond what the count needs leave at once.
 __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int64_t first, int64_t n, uint64_t seed,
                                                    Requests R, double* shower_energy, unsigned long long* counters,
-                                                   const double* sig_lds_src) {
+                                                   const double* sig_lds_src, const int64_t* range) {
+  if (range) {
+    first = range[0];
+    n = range[1];
+    if ((int64_t)blockIdx.x * 4 >= (n + 63) / 64) return;
+  }
   HIP_DYNAMIC_SHARED(double, s_sig)
   for (int i = threadIdx.x; i < T.lds_doubles; i += blockDim.x) s_sig[i] = sig_lds_src[i];
   __syncthreads();
@@ -704,23 +739,44 @@ struct EmitLaunch {
   int64_t total;
   const float* x_out;
   const int32_t* trials_out;
+  const unsigned int* dcount;  // non-null: per-process counts on the device (seg/total unused)
 };
 
+// Grid-stride over the requests of the sweep (all lanes of a wave take part in
+// every pass: the slot allocation below is wave-aggregated).
 __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunch io, uint64_t seed,
                                               unsigned long long* store_count, double* shower_energy,
                                               unsigned long long* stats) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t total = io.total;
+  if (io.dcount) {
+    total = 0;
+    for (int q = 0; q < PETITE_NPROC; ++q) total += io.dcount[q];
+  }
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x; r0 < total; r0 += stride) {
+  const int64_t r = r0 + threadIdx.x;
   const int lane = threadIdx.x & 63;
   bool keep[2] = {false, false};
   V4 lab[2];
   int cpid[2] = {0, 0};
   int64_t slot = 0;
   int proc = 0;
-  if (r < io.total) {
+  if (r < total) {
     int p = 0;
-    while (r >= io.seg[p + 1]) ++p;
+    int64_t start = 0;
+    if (io.dcount) {
+      int64_t acc = io.dcount[0];
+      while (r >= acc) {
+        start = acc;
+        ++p;
+        acc += io.dcount[p];
+      }
+    } else {
+      while (r >= io.seg[p + 1]) ++p;
+      start = io.seg[p];
+    }
     proc = p;
-    const int64_t oidx = (int64_t)p * io.req_cap + (r - io.seg[p]);
+    const int64_t oidx = (int64_t)p * io.req_cap + (r - start);
     slot = io.req_slot[oidx];
     const double E = io.req_E[oidx];
     const int pid = S.pid[slot];
@@ -805,6 +861,29 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
       ++dst;
     }
   }
+  }
+}
+
+// Start of a device-driven sweep: the range is the particles appended since
+// the last sweep (at most `bound`, the rest waits for the next sweep), and the
+// per-sweep counters are cleared.  range = {first, n, end}.
+__global__ void k_sweep_setup(unsigned long long* ctr, int64_t* range, unsigned int* req_count, int64_t capacity,
+                              int64_t bound) {
+  const int t = threadIdx.x;
+  if (t < PETITE_NPROC) req_count[t] = 0u;
+  if (t <= C_TRANSPORT) ctr[t] = 0ull;  // draw refill and transport refill counters
+  if (t == 0) {
+    const int64_t first = range[2];
+    int64_t now = (int64_t)ctr[C_STORE];
+    if (now > capacity) now = capacity;
+    int64_t n = now - first;
+    if (n > bound) n = bound;
+    if (n < 0) n = 0;
+    range[0] = first;
+    range[1] = n;
+    range[2] = first + n;
+    if (n > 0) ctr[C_SWEEPS] += 1ull;
+  }
 }
 
 #include "dark.h"
@@ -837,6 +916,9 @@ struct PetiteCtx {
   hipStream_t dstream[PETITE_NPROC] = {};
   hipEvent_t dfork = nullptr;
   hipEvent_t djoin[PETITE_NPROC] = {};
+  // device-driven sweeps: {first, n, end} of the current sweep, and timing events
+  int64_t* drange = nullptr;
+  std::vector<hipEvent_t> tev;
   int n_cu = 256;
   double tbar[PETITE_NPROC];  // running mean of candidates per draw, per process
   // dark sector
@@ -883,6 +965,11 @@ static int ensure_entries(PetiteCtx* c, int64_t cap) {
   return PETITE_OK;
 }
 
+// Device-driven sweeps: at most kSweepBound particles per sweep (request
+// scratch scales with it) and kSweepGroup sweeps queued per host check.
+constexpr int64_t kSweepBound = (int64_t)1 << 23;
+constexpr int kSweepGroup = 16;
+
 static int ensure_requests(PetiteCtx* c, int64_t cap) {
   if (cap <= c->req_cap) return PETITE_OK;
   cap = std::max<int64_t>(cap, 1024);
@@ -906,33 +993,20 @@ static int ensure_requests(PetiteCtx* c, int64_t cap) {
   return PETITE_OK;
 }
 
-// Launch plan of the per-process draw kernels (they run one after the other
-// on the stream, each with the whole GPU): one lane per request while there
-// are at least four requests per lane; below that, groups of G lanes share
-// a request, sized so the GPU stays filled and each group sees a few requests.
 static void plan_draw(PetiteCtx* c, const unsigned int* rq, DrawLaunch& io, int64_t& blocks_total) {
-  const int full_blocks = c->n_cu * 4;  // 4-wave blocks at 4 waves per SIMD
-  const double full_lanes = 256.0 * full_blocks;
   io.block_start[0] = 0;
+  io.full_blocks = c->n_cu * 4;  // 4-wave blocks at 4 waves per SIMD
   for (int p = 0; p < PETITE_NPROC; ++p) {
-    int b = 0, lg = 0;
+    int b, lg;
     io.count[p] = (int32_t)rq[p];
-    const double cnt = (double)rq[p];
-    if (rq[p]) {
-      if (cnt >= 4.0 * full_lanes) {
-        b = full_blocks;
-      } else {
-        const double want = 4.0 * full_lanes / cnt;
-        while ((1 << lg) < want && lg < 6) ++lg;
-        const double lanes = std::min(full_lanes, cnt * (double)(1 << lg));
-        b = std::max(1, (int)std::ceil(lanes / 256.0));
-      }
-    }
+    draw_plan((double)rq[p], io.full_blocks, lg, b);
     io.lgG[p] = lg;
     io.block_start[p + 1] = io.block_start[p] + b;
   }
   blocks_total = io.block_start[PETITE_NPROC];
 }
+
+static int ensure_draw_streams(PetiteCtx* c);
 
 // The per-process kernels are independent: with more than one, each runs on
 // its own stream so small processes overlap the large ones.
@@ -942,12 +1016,9 @@ static int launch_draws(PetiteCtx* c, const DrawLaunch& io, uint64_t seed, hipSt
     const int b = io.block_start[p + 1] - io.block_start[p];
     if (b > 0 && io.count[p] > 0 && c->desc.grid[p].n_E > 0) active[na++] = p;
   }
-  if (na > 1 && !c->dfork) {
-    HIP_TRY(hipEventCreateWithFlags(&c->dfork, hipEventDisableTiming));
-    for (int p = 0; p < PETITE_NPROC; ++p) {
-      HIP_TRY(hipStreamCreateWithFlags(&c->dstream[p], hipStreamNonBlocking));
-      HIP_TRY(hipEventCreateWithFlags(&c->djoin[p], hipEventDisableTiming));
-    }
+  if (na > 1) {
+    const int rc = ensure_draw_streams(c);
+    if (rc) return rc;
   }
   if (na > 1) HIP_TRY(hipEventRecord(c->dfork, s));
   for (int i = 0; i < na; ++i) {
@@ -963,6 +1034,34 @@ static int launch_draws(PetiteCtx* c, const DrawLaunch& io, uint64_t seed, hipSt
     if (na > 1) HIP_TRY(hipEventRecord(c->djoin[p], sp));
   }
   for (int i = 0; na > 1 && i < na; ++i) HIP_TRY(hipStreamWaitEvent(s, c->djoin[active[i]], 0));
+  return PETITE_OK;
+}
+
+static int ensure_draw_streams(PetiteCtx* c) {
+  if (c->dfork) return PETITE_OK;
+  HIP_TRY(hipEventCreateWithFlags(&c->dfork, hipEventDisableTiming));
+  for (int p = 0; p < PETITE_NPROC; ++p) {
+    HIP_TRY(hipStreamCreateWithFlags(&c->dstream[p], hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&c->djoin[p], hipEventDisableTiming));
+  }
+  return PETITE_OK;
+}
+
+// Device-planned draws of the shower pipeline: every SM process with a grid
+// gets a whole-GPU launch on its stream; blocks the count does not need leave.
+static int launch_draws_device(PetiteCtx* c, const DrawLaunch& io, uint64_t seed, hipStream_t s) {
+  int rc = ensure_draw_streams(c);
+  if (rc) return rc;
+  HIP_TRY(hipEventRecord(c->dfork, s));
+  for (int p = 0; p < 8; ++p) {
+    if (c->desc.grid[p].n_E <= 0) continue;
+    HIP_TRY(hipStreamWaitEvent(c->dstream[p], c->dfork, 0));
+    hipLaunchKernelGGL(kDrawKernels[p], dim3((unsigned)io.full_blocks), dim3(256), 0, c->dstream[p], c->dev, io, seed);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(c->djoin[p], c->dstream[p]));
+  }
+  for (int p = 0; p < 8; ++p)
+    if (c->desc.grid[p].n_E > 0) HIP_TRY(hipStreamWaitEvent(s, c->djoin[p], 0));
   return PETITE_OK;
 }
 
@@ -1063,6 +1162,9 @@ void petite_ctx_destroy(PetiteCtx* c) {
   for (hipEvent_t& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->dfork) (void)hipEventDestroy(c->dfork);
+  for (hipEvent_t e : c->tev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->drange) (void)hipFree(c->drange);
   for (int p = 0; p < PETITE_NPROC; ++p) {
     if (c->djoin[p]) (void)hipEventDestroy(c->djoin[p]);
     if (c->dstream[p]) (void)hipStreamDestroy(c->dstream[p]);
@@ -1167,109 +1269,113 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   hipStream_t s = (hipStream_t)stream;
   DevStore S{(float4*)store->p0, (float4*)store->r0, (float4*)store->pf, (float4*)store->rf, store->pid, store->parent,
              store->meta, store->shower, store->sid, store->capacity};
+  std::memset(stats, 0, sizeof(*stats));
+  // A sweep transports the particles appended since the previous one (at
+  // most `bound` of them), draws their interactions and appends the children.
+  // Sizes live on the device: the host queues kSweepGroup sweeps at a time
+  // and reads the counters once per group.
+  const int64_t bound = std::min<int64_t>(store->capacity, kSweepBound);
+  int rc = ensure_requests(c, bound);
+  if (rc) return rc;
+  if (!c->drange) HIP_TRY(hipMalloc(&c->drange, 4 * sizeof(int64_t)));
   unsigned long long* ctr = c->counters;
   HIP_TRY(hipMemsetAsync(ctr, 0, C_N * sizeof(unsigned long long), s));
+  HIP_TRY(hipMemsetAsync(c->drange, 0, 4 * sizeof(int64_t), s));
   const unsigned long long n0 = (unsigned long long)n;
   HIP_TRY(hipMemcpyAsync(ctr + C_STORE, &n0, sizeof(n0), hipMemcpyHostToDevice, s));
   hipLaunchKernelGGL(k_init_primaries, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, prim, n);
   HIP_TRY(hipGetLastError());
-  int64_t first = 0, count = n, written = n, iters = 0;
-  double t_tr = 0.0, t_dr = 0.0, t_em = 0.0;
-  int64_t l_tr = 0, l_dr = 0, l_em = 0;
-  std::memset(stats, 0, sizeof(*stats));
-  const int64_t max_waves = (int64_t)c->n_cu * 20;
-  struct {
-    unsigned int rq[PETITE_NPROC];
-  } hrq;
-  unsigned long long ptr[2 * PETITE_NPROC];
-  while (count > 0) {
-    int rc = ensure_requests(c, count);
-    if (rc) return rc;
-    Requests R{c->req_slot, c->req_E, c->req_cap, c->req_count};
-    HIP_TRY(hipMemsetAsync(c->req_count, 0, PETITE_NPROC * sizeof(unsigned int), s));
-    HIP_TRY(hipMemsetAsync(ctr, 0, 17 * sizeof(unsigned long long), s));  // refill counters
-    const int64_t tw = std::min<int64_t>((count + 63) / 64, max_waves);
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], s));
-    hipLaunchKernelGGL(k_transport, dim3((unsigned)((tw + 3) / 4)), dim3(256), c->dev.lds_doubles * sizeof(double), s,
-                       c->dev, S, first, count, seed, R, shower_energy, ctr, c->dsig_lds);
-    HIP_TRY(hipGetLastError());
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], s));
-    HIP_TRY(hipMemcpyAsync(hrq.rq, c->req_count, sizeof(hrq.rq), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(ptr, ctr + C_PTRIALS, sizeof(ptr), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    for (int p = 0; p < PETITE_NPROC; ++p)
-      if (ptr[PETITE_NPROC + p] > 0) c->tbar[p] = (double)ptr[p] / (double)ptr[PETITE_NPROC + p];
-    DrawLaunch io{};
-    io.req_slot = c->req_slot;
-    io.req_E = c->req_E;
-    io.sid_src = store->sid;
-    io.req_cap = c->req_cap;
-    io.explicit_out = 0;
-    io.x_out = c->res_x;
-    io.trials_out = c->res_trials;
-    io.w_out = c->res_w;
-    io.counters = ctr;
-    int64_t blocks = 0;
-    plan_draw(c, hrq.rq, io, blocks);
-    EmitLaunch em{};
-    em.req_slot = c->req_slot;
-    em.req_E = c->req_E;
-    em.req_cap = c->req_cap;
-    em.seg[0] = 0;
-    for (int p = 0; p < PETITE_NPROC; ++p) em.seg[p + 1] = em.seg[p] + hrq.rq[p];
-    em.total = em.seg[PETITE_NPROC];
-    em.x_out = c->res_x;
-    em.trials_out = c->res_trials;
-    if (em.total > 0) {
-      if (c->timing) HIP_TRY(hipEventRecord(c->ev[2], s));
-      rc = launch_draws(c, io, seed, s);
-      if (rc) return rc;
-      if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], s));
-      hipLaunchKernelGGL(k_emit, dim3((unsigned)((em.total + 255) / 256)), dim3(256), 0, s, c->dev, S, em, seed,
-                         ctr + C_STORE, shower_energy, ctr + C_STATS);
-      HIP_TRY(hipGetLastError());
-      if (c->timing) HIP_TRY(hipEventRecord(c->ev[4], s));
+  Requests R{c->req_slot, c->req_E, c->req_cap, c->req_count};
+  DrawLaunch io{};
+  io.req_slot = c->req_slot;
+  io.req_E = c->req_E;
+  io.sid_src = store->sid;
+  io.req_cap = c->req_cap;
+  io.explicit_out = 0;
+  io.x_out = c->res_x;
+  io.trials_out = c->res_trials;
+  io.w_out = c->res_w;
+  io.counters = ctr;
+  io.dcount = c->req_count;
+  io.full_blocks = c->n_cu * 2;  // per process: the processes run side by side
+  EmitLaunch em{};
+  em.req_slot = c->req_slot;
+  em.req_E = c->req_E;
+  em.req_cap = c->req_cap;
+  em.x_out = c->res_x;
+  em.trials_out = c->res_trials;
+  em.dcount = c->req_count;
+  const unsigned tr_blocks = (unsigned)(c->n_cu * 5);  // 20 waves per CU
+  const unsigned em_blocks = (unsigned)(c->n_cu * 8);
+  const size_t lds = (size_t)c->dev.lds_doubles * sizeof(double);
+  if (c->timing) {
+    while ((int)c->tev.size() < 4 * kSweepGroup) {
+      hipEvent_t e;
+      HIP_TRY(hipEventCreate(&e));
+      c->tev.push_back(e);
     }
-    unsigned long long now = 0;
-    HIP_TRY(hipMemcpyAsync(&now, ctr + C_STORE, sizeof(now), hipMemcpyDeviceToHost, s));
+  }
+  double t_tr = 0.0, t_dr = 0.0, t_em = 0.0;
+  int64_t queued = 0;
+  unsigned long long h_store = n0;
+  int64_t h_range[3] = {0, 0, 0};
+  while (true) {
+    for (int g = 0; g < kSweepGroup; ++g) {
+      hipLaunchKernelGGL(k_sweep_setup, dim3(1), dim3(64), 0, s, ctr, c->drange, c->req_count, store->capacity, bound);
+      HIP_TRY(hipGetLastError());
+      if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 0], s));
+      hipLaunchKernelGGL(k_transport, dim3(tr_blocks), dim3(256), lds, s, c->dev, S, (int64_t)0, (int64_t)0, seed, R,
+                         shower_energy, ctr, c->dsig_lds, (const int64_t*)c->drange);
+      HIP_TRY(hipGetLastError());
+      if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 1], s));
+      rc = launch_draws_device(c, io, seed, s);
+      if (rc) return rc;
+      if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 2], s));
+      hipLaunchKernelGGL(k_emit, dim3(em_blocks), dim3(256), 0, s, c->dev, S, em, seed, ctr + C_STORE, shower_energy,
+                         ctr + C_STATS);
+      HIP_TRY(hipGetLastError());
+      if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 3], s));
+    }
+    queued += kSweepGroup;
+    HIP_TRY(hipMemcpyAsync(&h_store, ctr + C_STORE, sizeof(h_store), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(h_range, c->drange, sizeof(h_range), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (c->timing) {
-      float ms = 0.0f;
-      HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-      t_tr += ms;
-      ++l_tr;
-      if (em.total > 0) {
-        HIP_TRY(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+      for (int g = 0; g < kSweepGroup; ++g) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->tev[4 * g + 0], c->tev[4 * g + 1]));
+        t_tr += ms;
+        HIP_TRY(hipEventElapsedTime(&ms, c->tev[4 * g + 1], c->tev[4 * g + 2]));
         t_dr += ms;
-        ++l_dr;
-        HIP_TRY(hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
+        HIP_TRY(hipEventElapsedTime(&ms, c->tev[4 * g + 2], c->tev[4 * g + 3]));
         t_em += ms;
-        ++l_em;
       }
     }
-    ++iters;
-    first = written;
-    count = (int64_t)now - written;
-    written = (int64_t)now;
-    if (written > store->capacity) break;
+    if ((int64_t)h_store > store->capacity) break;  // children were dropped: report the size reached
+    if (h_range[1] == 0 && (int64_t)h_store == h_range[2]) break;  // the last sweep found nothing to do
+    if (queued > 1000000) return fail(PETITE_ERR_HIP, "petite_run_showers: sweep count runaway");
   }
-  unsigned long long hs[ST_N];
+  unsigned long long hs[ST_N], sweeps = 0;
   HIP_TRY(hipMemcpyAsync(hs, ctr + C_STATS, sizeof(hs), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(&sweeps, ctr + C_SWEEPS, sizeof(sweeps), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
+  const int64_t written = (int64_t)h_store;
   stats->n_particles = written;
   stats->n_steps = (int64_t)hs[ST_STEPS];
   stats->n_draws = (int64_t)hs[ST_DRAWS];
   stats->n_trials = (int64_t)hs[ST_TRIALS];
   stats->n_maxf_viol = (int64_t)hs[ST_VIOL];
   stats->n_no_sample = (int64_t)hs[ST_NOSAMPLE];
-  stats->n_iterations = iters;
+  stats->n_iterations = (int64_t)sweeps;
   stats->n_dropped = (int64_t)hs[ST_DROPPED];
   stats->ms_transport = t_tr;
   stats->ms_draw = t_dr;
   stats->ms_emit = t_em;
-  stats->launches_transport = l_tr;
-  stats->launches_draw = l_dr;
-  stats->launches_emit = l_em;
+  // sweeps with work (the trailing empty sweeps of the last group add a few
+  // microseconds to the times but no launches worth counting)
+  stats->launches_transport = (int64_t)sweeps;
+  stats->launches_draw = (int64_t)sweeps;
+  stats->launches_emit = (int64_t)sweeps;
   if (written > store->capacity)
     return fail(PETITE_ERR_CAPACITY, "petite_run_showers: store capacity exceeded; n_particles holds the size reached");
   if (hs[ST_NOSAMPLE]) return fail(PETITE_ERR_NO_SAMPLE, "petite_run_showers: No Sample Found within max_trials");
