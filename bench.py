@@ -10,11 +10,10 @@ its descendants to the end (PETITE's generate_shower for each primary).
 Prints one JSON line: particle-steps/s (value), showers/s, the roofline of
 the dominant kernel (from HIP events on the run's stream) and a CPU baseline
 (the float64 oracle, one core, on a bounded sample of the same primaries).
-Two dark workloads follow under "dark_forward" (20,000 e- at 10 GeV in
-graphite, m_V = 1 MeV: vectors/s) and "dark_yield_scan" (a fixed record of
-50,000 e- showers in lead, yield and d/dm_V at 32 masses: gradient
-evaluations/s and the gradient/forward cost ratio), each with its CPU
-baseline.
+The dark-yield scan follows under "dark_yield_scan" (a fixed record of 50,000
+e- showers in lead, yield and d/dm_V at 32 masses: gradient evaluations/s and
+the gradient/forward cost ratio) with its CPU baseline; --dark-forward adds
+"dark_forward" (20,000 e- at 10 GeV in graphite, m_V = 1 MeV: vectors/s).
 """
 import argparse
 import json
@@ -38,6 +37,21 @@ FP32_PEAK_TFLOPS = 157.3
 # writes a 64-byte live record; an emitted particle writes a 96-byte record;
 # a primary is read once (32 bytes)
 B_STEP, B_OUT, B_PRIM = 128, 96, 32
+
+
+def measured_traffic(kernel, alg_bytes_per_launch):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE measurement of this workload (profiles/traffic.json, made by the
+    command it records), with the algorithmic bytes per launch beside it."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)[kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return {"bytes_per_launch": t["bytes_per_launch"], "fetch_bytes": t["fetch_bytes"],
+            "write_bytes": t["write_bytes"], "over_algorithmic": t["bytes_per_launch"] / alg_bytes_per_launch,
+            "source": "profiles/traffic.json"}
 
 
 def make_primaries(n=N_PRIMARIES, seed=SEED):
@@ -92,7 +106,9 @@ def main():
     ap.add_argument("--primaries", type=int, default=N_PRIMARIES)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--no-dark", action="store_true", help="skip the dark-vector and dark-yield workloads")
+    ap.add_argument("--no-dark", action="store_true", help="skip the dark-yield workload")
+    ap.add_argument("--dark-forward", action="store_true",
+                    help="also run the dark-vector workload (20,000 e- at 10 GeV, m_V = 1 MeV)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): showers are independent, so a multi-GPU "
@@ -154,6 +170,8 @@ def main():
                 "frac": ach / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "k_draw",
                 "per_unit": f"{FLOPS_PER_CANDIDATE} flop per candidate (vector ALU; no MFMA used)"}
     roof["avg_launch_ms"] = kern[dom][0] / max(kern[dom][1], 1)
+    if roof["bound"] == "hbm":
+        roof["traffic"] = measured_traffic(roof["kernel"], alg / max(kern[dom][1], 1))
     kernels = {k: {"ms_total": v[0] / K, "launches": v[1] // K} for k, v in kern.items()}
 
     cpu = None
@@ -162,7 +180,7 @@ def main():
         cpu = cpu_baseline(prim, maxf, args.cpu_budget)
 
     dark_out = yield_out = None
-    if not args.no_dark:
+    if args.dark_forward:
         del sh, ctx
         torch.cuda.empty_cache()
         dark_out, ds, rec, dark = bench_dark_forward()
@@ -171,6 +189,7 @@ def main():
             dark_out["cpu_baseline"] = cpu_dark_baseline(ds, rec)
             dark_out["gpu_over_cpu_vectors"] = dark_out["vectors_per_s"] / dark_out["cpu_baseline"]["value"]
         del ds, rec, dark
+    if not args.no_dark:
         torch.cuda.empty_cache()
         yield_out, ds, rec, model = bench_yield()
         print(f"yield: {yield_out['sm_records']} records, grad/forward {yield_out['grad_over_forward']:.2f}",
@@ -223,7 +242,7 @@ YIELD_N, YIELD_MAT = 50_000, "lead"
 YIELD_MASSES = np.geomspace(1e-3, 0.1, 32)
 # algorithmic bytes: the dark selection reads a 64-byte SM record and the
 # emission writes a 48-byte vector record; the yield pass reads pid, E0 and
-# the weight (12 bytes) of every record once per batch of 8 masses
+# the weight (12 bytes) of every record once per batch of 16 masses
 B_DARK_SM, B_DARK_V, B_YIELD = 64, 48, 12
 
 
@@ -318,7 +337,7 @@ def bench_yield(reps=5):
     Ym, _ = model.evaluate(rec, mv * (1 - 1e-3), grad=False)
     fd = (Yp - Ym) / (2e-3 * mv)
     fd_err = float(np.max(np.abs(fd - dY[inner]) / np.maximum(np.abs(dY[inner]), 1e-300)))
-    passes = math.ceil(len(YIELD_MASSES) / 8)
+    passes = math.ceil(len(YIELD_MASSES) / 16)
     alg = B_YIELD * rec.n * passes
     out = {"workload": f"{YIELD_N} e- showers, E log-uniform [1,20] GeV, {YIELD_MAT}, cutoff {CUTOFF} GeV; "
                        f"32 m_V in [1,100] MeV; eps^2 enters linearly",
@@ -327,7 +346,7 @@ def bench_yield(reps=5):
            "scans_per_s": 1.0 / t_g, "grad_evals_per_s": len(YIELD_MASSES) / t_g,
            "fd_max_rel_err": fd_err, "fd_points": len(inner),
            "roofline": {"bound": "hbm", "achieved": alg / t_g / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "per_unit": "12 B per record per 8-mass pass (host-timed, includes launch and readback)"},
+                        "per_unit": "12 B per record per 16-mass pass (host-timed, includes launch and readback)"},
            "yield_per_eps2_at_10MeV": float(np.interp(0.01, YIELD_MASSES, Y))}
     return out, ds, rec, model
 

@@ -183,6 +183,11 @@ typedef struct {
   double eps2;                         /* g_e^2 / (4 pi alpha) = kinetic_mixing^2 */
   double E_res, E_C;                   /* bound-electron thresholds (dark_shower.py:234-235) */
   double min_brem, min_ann, min_comp, min_muon_brem; /* _minimum_calculable_dark_energy (243-245) */
+  /* geometric weight grids: ln E[0] and (n-1)/ln(E[n-1]/E[0]) (0: search the grid);
+   * DarkComp: log10 E[0] and (n-1)/(log10 E[n-1] - log10 E[0]) of its uniform log grid */
+  double w_lnE0[PETITE_DARK_FAMILIES];
+  double w_inv_dlog[PETITE_DARK_FAMILIES];
+  double comp_l0, comp_inv_dl;
 } PetiteDarkDesc;
 
 /* Dark vectors written by petite_dark_reweight, one entry per (SM particle,

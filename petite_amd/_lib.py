@@ -70,7 +70,9 @@ class DarkDesc(C.Structure):
     _fields_ = [("w_n", C.c_int32 * 4), ("w_off", C.c_int64 * 4), ("d_n", C.c_int32 * 4), ("d_off", C.c_int64 * 4),
                 ("comp_n", C.c_int32), ("n_active", C.c_int32), ("comp_off", C.c_int64), ("active", C.c_int32 * 8),
                 ("eps2", C.c_double), ("E_res", C.c_double), ("E_C", C.c_double), ("min_brem", C.c_double),
-                ("min_ann", C.c_double), ("min_comp", C.c_double), ("min_muon_brem", C.c_double)]
+                ("min_ann", C.c_double), ("min_comp", C.c_double), ("min_muon_brem", C.c_double),
+                ("w_lnE0", C.c_double * 4), ("w_inv_dlog", C.c_double * 4), ("comp_l0", C.c_double),
+                ("comp_inv_dl", C.c_double)]
 
 
 class DarkOut(C.Structure):

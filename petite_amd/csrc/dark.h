@@ -33,30 +33,45 @@ struct Entries {
   int64_t cap;
 };
 
+// Interval lo with x[lo] <= v < x[lo+1] (v inside [x0, x_{n-1})): an O(1)
+// guess g0 + (v' - v0) * inv on uniform grids (in v' = ln v or v itself),
+// corrected against the stored nodes; bisection when inv = 0.
+__device__ inline int find_interval(const double* x, int n, double v, float guess_arg, double v0, double inv) {
+  int lo;
+  if (inv > 0.0) {
+    lo = (int)((guess_arg - (float)v0) * (float)inv);
+    lo = lo < 0 ? 0 : (lo > n - 2 ? n - 2 : lo);
+    while (lo > 0 && x[lo] > v) --lo;
+    while (lo < n - 2 && x[lo + 1] <= v) ++lo;
+  } else {
+    lo = 0;
+    int hi = n - 1;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (x[mid] <= v) lo = mid; else hi = mid;
+    }
+  }
+  return lo;
+}
+
 // numpy.interp on an ascending grid (slope form), 0 outside [x0, x_{n-1}]
 // (scipy interp1d with fill_value=0, dark_shower.py:447-452).
-__device__ inline double interp_fill0(const double* x, const double* y, int n, double E) {
+__device__ inline double interp_fill0(const double* x, const double* y, int n, double E, double lnE0 = 0.0,
+                                      double inv_dlog = 0.0) {
   if (n <= 0 || !(E >= x[0]) || E > x[n - 1]) return 0.0;
   if (E == x[n - 1]) return y[n - 1];
-  int lo = 0, hi = n - 1;  // x[lo] <= E < x[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (x[mid] <= E) lo = mid; else hi = mid;
-  }
+  const int lo = find_interval(x, n, E, __logf((float)E), lnE0, inv_dlog);
   const double slope = (y[lo + 1] - y[lo]) / (x[lo + 1] - x[lo]);
   return slope * (E - x[lo]) + y[lo];
 }
 
-// interpolate1d in log10-log10 with fill -20 (dark_shower.py:31-46, 294-309).
-__device__ inline double loglog(const double* lx, const double* ly, int n, double E) {
-  const double l = log10(E);
+// interpolate1d in log10-log10 with fill -20 (dark_shower.py:31-46, 294-309);
+// l = log10(E).
+__device__ inline double loglog(const double* lx, const double* ly, int n, double l, double l0 = 0.0,
+                                double inv_dl = 0.0) {
   if (n <= 0 || l < lx[0] || l > lx[n - 1]) return 1.0e-20;
-  int lo = 0, hi = n - 1;
   if (l == lx[n - 1]) return pow(10.0, ly[n - 1]);
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (lx[mid] <= l) lo = mid; else hi = mid;
-  }
+  const int lo = find_interval(lx, n, l, (float)l, l0, inv_dl);
   const double slope = (ly[lo + 1] - ly[lo]) / (lx[lo + 1] - lx[lo]);
   return pow(10.0, slope * (l - lx[lo]) + ly[lo]);
 }
@@ -68,21 +83,45 @@ __device__ inline int family_of(int code, int pid) {
   return -1;
 }
 
-// GetBSMWeights without the kinetic-mixing factor applied by the caller.
+// Quantities of one SM particle shared by the weights of every mass.
+struct WeightArgs {
+  double E0;
+  float lf;    // logf(E0): guess of the geometric-grid interval
+  double l10;  // log10(E0) for the log-log DarkComp table
+  double den;  // n*sigma_PairProd + n*sigma_Comp at E0 (photons)
+};
+
+__device__ inline WeightArgs weight_args(const DevTables& T, int pid, double E0) {
+  WeightArgs a;
+  a.E0 = E0;
+  a.lf = __logf((float)E0);
+  a.l10 = pid == 22 ? log10(E0) : 0.0;
+  a.den = pid == 22 ? nsigma(T, PAIRPROD, E0) + nsigma(T, COMP, E0) : 0.0;
+  return a;
+}
+
+// GetBSMWeights (dark_shower.py:595-647), eps^2 included.
 __device__ inline double bsm_weight(const DevTables& T, const PetiteDarkDesc& D, const double* b, int pid, int code,
-                                    double E0) {
+                                    const WeightArgs& a) {
+  const double E0 = a.E0;
   if (code == DARKCOMP) {
     if (pid != 22 || E0 < D.min_comp || E0 < T.d->min_calc[2]) return 0.0;
     const double* lx = b + D.comp_off;
-    const double num = loglog(lx, lx + D.comp_n, D.comp_n, E0);
-    return D.eps2 * (num / (nsigma(T, PAIRPROD, E0) + nsigma(T, COMP, E0)));
+    const double num = loglog(lx, lx + D.comp_n, D.comp_n, a.l10, D.comp_l0, D.comp_inv_dl);
+    return D.eps2 * (num / a.den);
   }
   const int f = family_of(code, pid);
   if (f < 0) return 0.0;
   const double thr = (code == DARKBREM) ? D.min_brem : (code == DARKANN ? D.min_ann : D.min_muon_brem);
   if (E0 < thr) return 0.0;
   const double* x = b + D.w_off[f];
-  return D.eps2 * interp_fill0(x, x + D.w_n[f], D.w_n[f], E0);
+  const int n = D.w_n[f];
+  if (n <= 0 || !(E0 >= x[0]) || E0 > x[n - 1]) return 0.0;
+  const double* y = x + n;
+  if (E0 == x[n - 1]) return D.eps2 * y[n - 1];
+  const int lo = find_interval(x, n, E0, a.lf, D.w_lnE0[f], D.w_inv_dlog[f]);
+  const double slope = (y[lo + 1] - y[lo]) / (x[lo + 1] - x[lo]);
+  return D.eps2 * (slope * (E0 - x[lo]) + y[lo]);
 }
 
 // numpy's sum of a short row: sequential below 8 terms, else eight partial
@@ -164,10 +203,11 @@ __global__ __launch_bounds__(256) void k_dark_select(DevTables T, PetiteDarkDesc
     sid = S.sid[i];
   }
   const double E0 = (double)p0.x;
+  const WeightArgs wa = weight_args(T, pid, E0);
   unsigned long long n_zero = 0;
   for (int a = 0; a < D.n_active; ++a) {
     const int code = D.active[a];  // wave-uniform
-    const double wg = live ? bsm_weight(T, D, b, pid, code, E0) : 0.0;
+    const double wg = live ? bsm_weight(T, D, b, pid, code, wa) : 0.0;
     const bool want = wg > 0.0;
     const unsigned long long m = __ballot(want);
     if (!m) continue;
@@ -412,7 +452,7 @@ __global__ __launch_bounds__(256) void k_dark_emit(DevTables T, PetiteDarkDesc D
 }
 
 // ------------------------------------------------------ m_V-continuous yield --
-constexpr int kYieldBatch = 8;
+constexpr int kYieldBatch = 16;
 
 struct YieldTables {
   PetiteDarkDesc d[PETITE_MAX_MASSES];
@@ -431,8 +471,9 @@ struct YieldPoints {
 // One float64 pass over the record: acc[2k] += w W(m_k), acc[2k+1] += w dW/dm(m_k)
 // (GRAD = false: the yield alone).
 template <bool GRAD>
-__global__ __launch_bounds__(256) void k_dark_yield(DevTables T, YieldTables Y, DevStore S, int64_t n, YieldPoints P,
-                                                    double* acc) {
+__global__ __launch_bounds__(256) void k_dark_yield(DevTables T, const YieldTables* __restrict__ Yp, DevStore S,
+                                                    int64_t n, YieldPoints P, double* acc) {
+  const YieldTables& Y = *Yp;
   double y[kYieldBatch], dy[kYieldBatch];
 #pragma unroll
   for (int k = 0; k < kYieldBatch; ++k) y[k] = dy[k] = 0.0;
@@ -442,28 +483,34 @@ __global__ __launch_bounds__(256) void k_dark_yield(DevTables T, YieldTables Y, 
     const int pid = S.pid[i];
     const double E0 = (double)S.p0[i].x;
     const double w = (double)S.r0[i].w;
+    const WeightArgs wa = weight_args(T, pid, E0);
     for (int a = 0; a < d0.n_active; ++a) {
       const int code = d0.active[a];
       if (code == DARKCOMP ? pid != 22 : family_of(code, pid) < 0) continue;
-      double wj[PETITE_MAX_MASSES];
+      double wj[PETITE_MAX_MASSES], rj[PETITE_MAX_MASSES];
 #pragma unroll
       for (int j = 0; j < PETITE_MAX_MASSES; ++j)
-        wj[j] = j < Y.n_mass ? bsm_weight(T, Y.d[j], Y.b, pid, code, E0) : 0.0;
+        wj[j] = j < Y.n_mass ? bsm_weight(T, Y.d[j], Y.b, pid, code, wa) : 0.0;
+      // log ratio of each segment, formed once per particle for all masses
+#pragma unroll
+      for (int j = 0; j + 1 < PETITE_MAX_MASSES; ++j)
+        rj[j] = (j + 1 < Y.n_mass && wj[j] > 0.0 && wj[j + 1] > 0.0) ? log(wj[j + 1] / wj[j]) : 0.0;
+      rj[PETITE_MAX_MASSES - 1] = 0.0;
 #pragma unroll
       for (int k = 0; k < kYieldBatch; ++k) {
         if (k >= P.n) break;
         const int j = P.j[k];
-        double lo = 0.0, hi = 0.0;
+        double lo = 0.0, hi = 0.0, r = 0.0;
 #pragma unroll
         for (int jj = 0; jj < PETITE_MAX_MASSES; ++jj) {
           lo = jj == j ? wj[jj] : lo;
           hi = jj == j + 1 ? wj[jj] : hi;
+          r = jj == j ? rj[jj] : r;
         }
         if (j + 1 >= Y.n_mass) hi = lo;
         const double t = P.t[k];
         double W, dW;
         if (lo > 0.0 && hi > 0.0) {
-          const double r = log(hi / lo);
           W = lo * exp(t * r);
           dW = GRAD ? W * r * P.dtdm[k] : 0.0;
         } else {

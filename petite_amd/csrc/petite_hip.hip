@@ -44,6 +44,7 @@ enum CounterIdx {
   C_TRANSPORT = 16,           // transport refill counter
   C_STORE = 17,               // particles written to the store
   C_SWEEPS = 18,              // sweeps with work (device-driven showers)
+  C_MUONS = 19,               // primaries that are muons (muons only come from primaries)
   C_DARK_N = 20,              // dark entries written in the current pass
   C_DARK_STATS = 21,          // [3] drate rows summing to zero, vectors written, bound-electron momentum failures
   C_STATS = 32,               // [ST_N] run statistics
@@ -403,10 +404,11 @@ struct DevStore {
   int64_t capacity;
 };
 
-__global__ void k_init_primaries(DevStore S, const PetitePrimary* prim, int64_t n) {
+__global__ void k_init_primaries(DevStore S, const PetitePrimary* prim, int64_t n, unsigned long long* n_muons) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const PetitePrimary p = prim[i];
+  if ((p.pid == 13 || p.pid == -13) && n_muons) atomicAdd(n_muons, 1ull);
   S.p0[i] = make_float4((float)p.E, (float)p.px, (float)p.py, (float)p.pz);
   S.r0[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.weight);
   S.pf[i] = S.p0[i];
@@ -934,6 +936,7 @@ struct PetiteCtx {
   int32_t n_mass = 0;
   double masses[PETITE_MAX_MASSES] = {};
   dark::YieldTables ytab{};
+  dark::YieldTables* ytab_dev = nullptr;  // device copy read by the kernel
   double* yield_blob = nullptr;
   double* yacc = nullptr;
 };
@@ -1049,19 +1052,20 @@ static int ensure_draw_streams(PetiteCtx* c) {
 
 // Device-planned draws of the shower pipeline: every SM process with a grid
 // gets a whole-GPU launch on its stream; blocks the count does not need leave.
-static int launch_draws_device(PetiteCtx* c, const DrawLaunch& io, uint64_t seed, hipStream_t s) {
+static int launch_draws_device(PetiteCtx* c, const DrawLaunch& io, uint64_t seed, hipStream_t s, bool muons) {
   int rc = ensure_draw_streams(c);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(c->dfork, s));
+  auto used = [&](int p) { return c->desc.grid[p].n_E > 0 && (muons || (p != MUONE && p != MUONBREM)); };
   for (int p = 0; p < 8; ++p) {
-    if (c->desc.grid[p].n_E <= 0) continue;
+    if (!used(p)) continue;
     HIP_TRY(hipStreamWaitEvent(c->dstream[p], c->dfork, 0));
     hipLaunchKernelGGL(kDrawKernels[p], dim3((unsigned)io.full_blocks), dim3(256), 0, c->dstream[p], c->dev, io, seed);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->djoin[p], c->dstream[p]));
   }
   for (int p = 0; p < 8; ++p)
-    if (c->desc.grid[p].n_E > 0) HIP_TRY(hipStreamWaitEvent(s, c->djoin[p], 0));
+    if (used(p)) HIP_TRY(hipStreamWaitEvent(s, c->djoin[p], 0));
   return PETITE_OK;
 }
 
@@ -1171,7 +1175,7 @@ void petite_ctx_destroy(PetiteCtx* c) {
   }
   for (void* p : {(void*)c->d64, (void*)c->d32, (void*)c->dmaxf, (void*)c->req_slot, (void*)c->req_E, (void*)c->res_x,
                   (void*)c->res_trials, (void*)c->res_w, (void*)c->req_count, (void*)c->counters, (void*)c->ddesc,
-                  (void*)c->dsig_lds, (void*)c->dark_blob, (void*)c->dyield, (void*)c->yield_blob, (void*)c->yacc})
+                  (void*)c->dsig_lds, (void*)c->dark_blob, (void*)c->dyield, (void*)c->yield_blob, (void*)c->yacc, (void*)c->ytab_dev})
     if (p) (void)hipFree(p);
   free_entries(c);
   delete c;
@@ -1283,8 +1287,13 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   HIP_TRY(hipMemsetAsync(c->drange, 0, 4 * sizeof(int64_t), s));
   const unsigned long long n0 = (unsigned long long)n;
   HIP_TRY(hipMemcpyAsync(ctr + C_STORE, &n0, sizeof(n0), hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_init_primaries, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, prim, n);
+  hipLaunchKernelGGL(k_init_primaries, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, S, prim, n, ctr + C_MUONS);
   HIP_TRY(hipGetLastError());
+  // muon processes can only be requested when a primary is a muon
+  unsigned long long n_muons = 0;
+  HIP_TRY(hipMemcpyAsync(&n_muons, ctr + C_MUONS, sizeof(n_muons), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  const bool muons = n_muons > 0;
   Requests R{c->req_slot, c->req_E, c->req_cap, c->req_count};
   DrawLaunch io{};
   io.req_slot = c->req_slot;
@@ -1328,7 +1337,7 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
                          shower_energy, ctr, c->dsig_lds, (const int64_t*)c->drange);
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 1], s));
-      rc = launch_draws_device(c, io, seed, s);
+      rc = launch_draws_device(c, io, seed, s, muons);
       if (rc) return rc;
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 2], s));
       hipLaunchKernelGGL(k_emit, dim3(em_blocks), dim3(256), 0, s, c->dev, S, em, seed, ctr + C_STORE, shower_energy,
@@ -1371,11 +1380,11 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   stats->ms_transport = t_tr;
   stats->ms_draw = t_dr;
   stats->ms_emit = t_em;
-  // sweeps with work (the trailing empty sweeps of the last group add a few
-  // microseconds to the times but no launches worth counting)
-  stats->launches_transport = (int64_t)sweeps;
-  stats->launches_draw = (int64_t)sweeps;
-  stats->launches_emit = (int64_t)sweeps;
+  // every queued sweep launched each stage once (n_iterations counts the
+  // sweeps with work; the trailing empty ones of the last group exit at once)
+  stats->launches_transport = queued;
+  stats->launches_draw = queued;
+  stats->launches_emit = queued;
   if (written > store->capacity)
     return fail(PETITE_ERR_CAPACITY, "petite_run_showers: store capacity exceeded; n_particles holds the size reached");
   if (hs[ST_NOSAMPLE]) return fail(PETITE_ERR_NO_SAMPLE, "petite_run_showers: No Sample Found within max_trials");
@@ -1551,6 +1560,8 @@ int petite_yield_setup(PetiteCtx* c, int32_t n_mass, const double* masses, const
   }
   c->ytab.n_mass = n_mass;
   c->ytab.b = c->yield_blob;
+  if (!c->ytab_dev) HIP_TRY(hipMalloc(&c->ytab_dev, sizeof(dark::YieldTables)));
+  HIP_TRY(hipMemcpy(c->ytab_dev, &c->ytab, sizeof(dark::YieldTables), hipMemcpyHostToDevice));
   return PETITE_OK;
 }
 
@@ -1595,11 +1606,11 @@ int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t
     HIP_TRY(hipMemsetAsync(c->yacc, 0, 2 * dark::kYieldBatch * sizeof(double), s));
     if (n_sm > 0) {
       if (dyield_host)
-        hipLaunchKernelGGL(dark::k_dark_yield<true>, dim3((unsigned)blocks), dim3(256), 0, s, c->dev, c->ytab, S, n_sm,
-                           P, c->yacc);
+        hipLaunchKernelGGL(dark::k_dark_yield<true>, dim3((unsigned)blocks), dim3(256), 0, s, c->dev,
+                           (const dark::YieldTables*)c->ytab_dev, S, n_sm, P, c->yacc);
       else
-        hipLaunchKernelGGL(dark::k_dark_yield<false>, dim3((unsigned)blocks), dim3(256), 0, s, c->dev, c->ytab, S,
-                           n_sm, P, c->yacc);
+        hipLaunchKernelGGL(dark::k_dark_yield<false>, dim3((unsigned)blocks), dim3(256), 0, s, c->dev,
+                           (const dark::YieldTables*)c->ytab_dev, S, n_sm, P, c->yacc);
       HIP_TRY(hipGetLastError());
     }
     double h[2 * dark::kYieldBatch];

@@ -22,7 +22,7 @@ reference does.
 import numpy as np
 
 from .constants import GeVsqcm2, alpha_em, cmtom, m_electron, m_muon
-from .tables import MaterialTables, _load
+from .tables import MaterialTables, _load, geometric
 
 DARK_PROCESSES = ["DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"]
 _GL_X, _GL_W = np.polynomial.legendre.leggauss(8)
@@ -413,6 +413,7 @@ def build_dark_blob(dt: DarkTables, active_codes):
             raise ValueError(f"{name}: weight energies must ascend")
         desc.w_n[f] = len(W)
         desc.w_off[f] = put(np.concatenate([W[:, 0], W[:, 1]])) if len(W) else 0
+        desc.w_lnE0[f], desc.w_inv_dlog[f] = geometric(W[:, 0]) if len(W) > 1 else (0.0, 0.0)
         Es, rows, cnt = dt.D[name]
         Es = np.asarray(Es, dtype=np.float64)
         order = np.argsort(Es, kind="stable")
@@ -426,6 +427,9 @@ def build_dark_blob(dt: DarkTables, active_codes):
     ll = dt.ns["DarkComp"]
     desc.comp_n = len(ll.lx)
     desc.comp_off = put(np.concatenate([ll.lx, ll.ly]))
+    d_lx = np.diff(ll.lx)
+    if len(ll.lx) > 1 and np.all(d_lx > 0) and np.max(np.abs(d_lx / d_lx.mean() - 1.0)) < 1e-9:
+        desc.comp_l0, desc.comp_inv_dl = float(ll.lx[0]), float((len(ll.lx) - 1) / (ll.lx[-1] - ll.lx[0]))
     codes = list(active_codes)
     if len(codes) > 8:
         raise ValueError("at most 8 active dark processes")

@@ -8,7 +8,7 @@ interpolates each particle's weight between the trained masses m_j < m_{j+1}:
 W_j^(1-t) W_{j+1}^t (t = ln(m_V/m_j) / ln(m_{j+1}/m_j)) where both are positive,
 (1-t) W_j + t W_{j+1} otherwise; at a trained mass it equals the weight sum of
 DarkShower.generate_dark_shower's entries.  One float64 pass of the
-k_dark_yield kernel returns Y and dY/dm_V for up to 8 masses at a time;
+k_dark_yield kernel returns Y and dY/dm_V for up to 16 masses at a time;
 dY/d(eps^2) = Y.  ``DarkYield.__call__`` wraps it in a torch.autograd.Function.
 """
 import ctypes as C
