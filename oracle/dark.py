@@ -233,8 +233,11 @@ class OracleDark:
         self._grids = {}
         for P in DARK_CODE:
             gk = f"{P}__{key}"
-            if f"E__{gk}" in z.files:
-                self._grids[P] = Grid(z[f"E__{gk}"], z[f"nodes__{gk}"], z[f"nodeoff__{gk}"], z[f"nodecnt__{gk}"])
+            src = z
+            if f"E__{gk}" not in z.files and table_set != "100GeV":
+                src, _ = load_pack("dark", "100GeV")  # maps missing from the 400 GeV set
+            if f"E__{gk}" in src.files:
+                self._grids[P] = Grid(src[f"E__{gk}"], src[f"nodes__{gk}"], src[f"nodeoff__{gk}"], src[f"nodecnt__{gk}"])
 
     def _ii(self, P, E):
         xg, yg = self.II[P]

@@ -223,11 +223,21 @@ class DarkTables:
         self._build_survival()
         self._build_weights(z, key)
         self.grids = {}
+        fallback = None
         for P in DARK_PROCESSES:
             gk = f"{P}__{key}"
-            if f"E__{gk}" in z.files:
-                self.grids[P] = (np.array(z[f"E__{gk}"]), np.array(z[f"nodes__{gk}"], dtype=np.float32),
-                                 np.array(z[f"nodecnt__{gk}"]))
+            src = z
+            if f"E__{gk}" not in z.files and sm.table_set != "100GeV":
+                # the 400 GeV set ships no DarkBrem/DarkMuonBrem maps: use the
+                # 100 GeV maps of the same mass (their variables are
+                # dimensionless; energies above the last grid energy take the
+                # last map, as LU_Key clamps)
+                if fallback is None:
+                    fallback, _ = _load("dark", "100GeV")
+                src = fallback
+            if f"E__{gk}" in src.files:
+                self.grids[P] = (np.array(src[f"E__{gk}"]), np.array(src[f"nodes__{gk}"], dtype=np.float32),
+                                 np.array(src[f"nodecnt__{gk}"]))
         # n*sigma of the dark processes for the device (log-log -> tabulate as given)
         self.nsigma_tab = {}
 
@@ -255,7 +265,8 @@ class DarkTables:
                  -13: ("MuonBrem", "MuonE")}[pid]
         E = np.asarray(E, dtype=np.float64)
         d = sum(self._II(P, Ei) - self._II(P, E) for P in chans)
-        out = np.exp(-d / self.dEdxT / cmtom)
+        with np.errstate(over="ignore"):
+            out = np.exp(-d / self.dEdxT / cmtom)
         return np.where((d < 0.0) | (E > Ei), 0.0, out)
 
     def mfp(self, pid, E):
