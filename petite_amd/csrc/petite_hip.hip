@@ -654,20 +654,25 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
         }
         wait = false;
       }
-      // wave-aggregated appends to the per-process request regions (SM processes)
+      // wave-aggregated appends to the per-process request regions (SM
+      // processes): lane pp reserves process pp's share, so the reservations
+      // of all processes are in flight together (one atomic round trip)
       if (__ballot(want >= 0)) {
+        unsigned long long mine = 0ull;
+        unsigned cnt = 0u;
+#pragma unroll
         for (int pp = 0; pp < 8; ++pp) {
           const unsigned long long mask = __ballot(want == pp);
-          if (!mask) continue;
-          const int leader = __ffsll((long long)mask) - 1;
-          unsigned base = 0;
-          if (lane == leader) base = atomicAdd(R.count + pp, (unsigned)__popcll(mask));
-          base = __shfl(base, leader);
-          if (want == pp) {
-            const unsigned idx = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
-            R.slot[(int64_t)pp * R.cap + idx] = (int32_t)slot;
-            R.E[(int64_t)pp * R.cap + idx] = E;
-          }
+          if (want == pp) mine = mask;
+          if (lane == pp) cnt = (unsigned)__popcll(mask);
+        }
+        unsigned base = 0u;
+        if (cnt) base = atomicAdd(R.count + lane, cnt);
+        base = __shfl(base, want >= 0 ? want : 0);
+        if (want >= 0) {
+          const unsigned idx = base + (unsigned)__popcll(mine & ((1ull << lane) - 1ull));
+          R.slot[(int64_t)want * R.cap + idx] = (int32_t)slot;
+          R.E[(int64_t)want * R.cap + idx] = E;
         }
       }
     }
