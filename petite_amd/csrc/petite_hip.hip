@@ -74,12 +74,15 @@ struct DrawLaunch {
   int32_t full_blocks;         // blocks of a whole-GPU launch (device plan)
 };
 
-// Launch plan of one process's draw kernel: one lane per request while there
-// are at least four requests per lane; below that, groups of 2^lg lanes share
-// a request, sized so the GPU stays filled and each group sees a few requests.
+// Launch plan of one process's draw kernel: groups of 2^lg lanes share a
+// request, at least 4 (measured on MI355X: 10 % faster than single lanes, whose
+// refills - request loads, LU_Key search, max_F - come four times as often),
+// more when there are fewer than four requests per group, sized so the GPU
+// stays filled and each group sees a few requests.
+constexpr int kMinGroupLog = 2;
 __host__ __device__ inline void draw_plan(double cnt, int full_blocks, int& lg, int& blocks) {
   const double full_lanes = 256.0 * full_blocks;
-  lg = 0;
+  lg = kMinGroupLog;
   blocks = 0;
   if (cnt <= 0.0) return;
   if (cnt >= 4.0 * full_lanes) {
@@ -190,9 +193,9 @@ __device__ inline double candidate_w(const DevTables& T, double E, const float* 
 // Groups of G = 2^lgG lanes work on one request: lane g of a group tests
 // candidate j + g, and the group takes the lowest accepted index, which is
 // the candidate the sequential loop of draw_sample accepts (candidate j
-// always uses Philox counter j).  G = 1 keeps every lane on its own request
-// and refills it as soon as it accepts; larger G cuts the latency of sweeps
-// with fewer requests than lanes.
+// always uses Philox counter j).  The result does not depend on G; a group
+// refills less often than a single lane, and larger G cuts the latency of
+// sweeps with fewer requests than lanes.
 template <int p>
 __device__ void draw_loop(const DevTables& T, const DrawLaunch& io, uint64_t seed, const int64_t count, const int lgG) {
   constexpr int ND = proc_ndim<p>();
