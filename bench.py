@@ -98,6 +98,21 @@ def cpu_baseline(prim, maxf, budget_s=20.0):
                       f"(oracle float64 restatement, per-candidate accept/reject)"}
 
 
+def copy_calibration():
+    """Three copies of a 256 MiB device buffer: a kernel of known traffic
+    (256 MiB read + 256 MiB written per copy) against which the rocprofv3
+    FETCH_SIZE / WRITE_SIZE readings of the shower kernels are scaled."""
+    import torch
+    n = 64 * 1024 * 1024
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copied {n * 4} bytes x3", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,7 +124,13 @@ def main():
     ap.add_argument("--no-dark", action="store_true", help="skip the dark-yield workload")
     ap.add_argument("--dark-forward", action="store_true",
                     help="also run the dark-vector workload (20,000 e- at 10 GeV, m_V = 1 MeV)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy a 256 MiB device buffer three times (the reference load for FETCH_SIZE and "
+                         "WRITE_SIZE counter scaling) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): showers are independent, so a multi-GPU "
               "run is a sharding of primaries with no exchange; see DESIGN.md", file=sys.stderr)
