@@ -61,6 +61,7 @@ struct DrawLaunch {
   const int32_t* req_slot;   // store slot (pipeline) or request index (explicit API)
   const double* req_E;
   const uint64_t* sid_src;   // stream ids, indexed by req_slot
+  const uint64_t* req_sid;   // non-null: stream ids beside the requests (used instead of sid_src)
   int64_t req_cap;
   int32_t count[PETITE_NPROC];
   int32_t block_start[PETITE_NPROC + 1];
@@ -229,7 +230,7 @@ __device__ void draw_loop(const DevTables& T, const DrawLaunch& io, uint64_t see
           const int32_t src = io.req_slot[ridx];
           oidx = io.explicit_out ? (int64_t)src : ridx;
           E = io.req_E[ridx];
-          sid = io.sid_src[src];
+          sid = io.req_sid ? io.req_sid[ridx] : io.sid_src[src];
           const int ie = lu_key(T, p, E);
           base = T.f32 + gd.node_off + (int64_t)ie * gd.stride;
           mf = T.maxf[p * T.maxE + ie] * T.d->maxF_fudge;
@@ -441,10 +442,36 @@ struct Requests {
   unsigned int* count;  // [PETITE_NPROC]
 };
 
+// What the draw and the emission need of the parent, written beside the
+// request by the transport ([PETITE_NPROC_SM][cap], request order), so that
+// neither kernel gathers from the store: requests are appended in completion
+// order, which scatters their slots over the sweep's range.
+struct ParentRecs {
+  uint64_t* sid;
+  float4* pf;    // final momentum (E, px, py, pz), as written to the store
+  float4* pos;   // final position xyz, weight
+  int2* info;    // (generation << 16 | pid & 0xFFFF, shower)
+};
+constexpr int kNprocSM = 8;
+
+__device__ inline int info_pid(int2 v) { return (int)(int16_t)(v.x & 0xFFFF); }
+__device__ inline int info_gen(int2 v) { return (v.x >> 16) & 0xFFFF; }
+
 // n*sigma tables in LDS.  Header per process p (4 doubles): ln E0,
 // (n-1)/ln(E_last/E0), offset of the energies, n; the energies, the values and
 // the interval slopes (y[i+1]-y[i])/(x[i+1]-x[i]) follow, so a lookup is
 // slope * (E - x[lo]) + y[lo], the arithmetic of scipy's interp1d.
+// Channels of a charged species' n*sigma sum, in the order of its terms
+// (shower.py:355-366), 4 bits each, 0xF = none; the sum is 0 without channels.
+__device__ inline uint32_t sum_channels(int pid) {
+  switch (pid) {
+    case 11: return BREM | (MOLLER << 4) | (0xFu << 8);
+    case -11: return BREM | (BHABHA << 4) | (ANN << 8);
+    case 13: case -13: return MUONBREM | (MUONE << 4) | (0xFu << 8);
+    default: return 0xFFFu;
+  }
+}
+
 struct LdsSigma {
   const double* base;
   // lf = logf(E), shared by the lookups of one energy (geometric-grid guess)
@@ -478,6 +505,17 @@ struct LdsSigma {
       case 13: case -13: return nsigma(MUONBREM, E, lf) + nsigma(MUONE, E, lf);
       default: return 0.0;
     }
+  }
+  // The same sums for a lane's charged species from its channel word (see
+  // sum_channels): every lane runs the same two lookups, positrons a third,
+  // so a wave that mixes species does not run each species' sum in turn.
+  __device__ inline double nsigma_sum_ch(uint32_t ch, double E) const {
+    if ((ch & 0xFu) == 0xFu) return 0.0;
+    const float lf = __logf((float)E);
+    double s = nsigma((int)(ch & 0xFu), E, lf) + nsigma((int)((ch >> 4) & 0xFu), E, lf);
+    const uint32_t c2 = (ch >> 8) & 0xFu;
+    if (c2 != 0xFu) s += nsigma((int)c2, E, lf);
+    return s;
   }
   __device__ inline double mfp(int pid, double E) const {
     const double s = nsigma_sum(pid, E);
@@ -542,8 +580,9 @@ constexpr int kBatchLanes = 32;
 // range (non-null): the sweep's first slot and count, set on the device by
 // k_sweep_setup; blocks beyond what the count needs leave at once.
 __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int64_t first, int64_t n, uint64_t seed,
-                                                   Requests R, double* shower_energy, unsigned long long* counters,
-                                                   const double* sig_lds_src, const int64_t* range) {
+                                                   Requests R, ParentRecs PR, double* shower_energy,
+                                                   unsigned long long* counters, const double* sig_lds_src,
+                                                   const int64_t* range) {
   if (range) {
     first = range[0];
     n = range[1];
@@ -558,10 +597,11 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
   // lane states: stepping (act), finished and parked (wait), empty (neither), no work left (done)
   bool act = false, wait = false, done = false;
   int64_t slot = 0;
-  int pid = 0, shower = 0;
+  int pid = 0, shower = 0, gen = 0;
+  uint32_t ch = 0xFFFu;  // channels of the lane's n*sigma sum
   uint64_t sid = 0;
   double m = 0.0, pme = 0.0, E = 0.0, p = 0.0, dep = 0.0, dz = 0.0;
-  float nx = 0.0f, ny = 0.0f, nz = 0.0f, rx = 0.0f, ry = 0.0f, rz = 0.0f;
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f, rx = 0.0f, ry = 0.0f, rz = 0.0f, wgt = 0.0f;
   uint32_t k = 0;
   bool closing = false;  // the next advance is the closing increment
   unsigned long long steps = 0;
@@ -580,6 +620,9 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
           pid = S.pid[slot];
           sid = S.sid[slot];
           shower = S.shower[slot];
+          gen = S.meta[slot] & 0xFFFF;
+          wgt = r0.w;
+          ch = sum_channels(pid);
           m = pid_mass(pid);
           pme = fmax(fmax(min_calc(T, pid), T.d->min_energy), m);
           E = (double)p0.x;
@@ -674,8 +717,13 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
         base = __shfl(base, want >= 0 ? want : 0);
         if (want >= 0) {
           const unsigned idx = base + (unsigned)__popcll(mine & ((1ull << lane) - 1ull));
-          R.slot[(int64_t)want * R.cap + idx] = (int32_t)slot;
-          R.E[(int64_t)want * R.cap + idx] = E;
+          const int64_t o = (int64_t)want * R.cap + idx;
+          R.slot[o] = (int32_t)slot;
+          R.E[o] = E;
+          PR.sid[o] = sid;
+          PR.pf[o] = make_float4((float)E, (float)(p * nx), (float)(p * ny), (float)(p * nz));
+          PR.pos[o] = make_float4(rx, ry, rz, wgt);
+          PR.info[o] = make_int2((gen << 16) | (pid & 0xFFFF), shower);
         }
       }
     }
@@ -694,7 +742,7 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
       //            dist = -mfp ln(1 + (exp(-dz/mfp) - 1) u) at the current energy;
       //            its scattering uses m_e (the reference omits m_lepton there).
       const bool in_close = closing;
-      const double S = L.nsigma_sum(pid, E);
+      const double S = L.nsigma_sum_ch(ch, E);
       const double mfp = S > 0.0 ? kCmToM / S : 1.0e12;
       const U4 u = draw_block(seed, sid, in_close ? 0u : k, in_close ? PURPOSE_CLOSE : PURPOSE_STEP);
       bool adv = true;
@@ -750,6 +798,7 @@ struct EmitLaunch {
   const float* x_out;
   const int32_t* trials_out;
   const unsigned int* dcount;  // non-null: per-process counts on the device (seg/total unused)
+  ParentRecs par;              // the parents' records, beside the requests
 };
 
 // Grid-stride over the requests of the sweep (all lanes of a wave take part in
@@ -771,6 +820,9 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
   int cpid[2] = {0, 0};
   int64_t slot = 0;
   int proc = 0;
+  int64_t oidx = 0;
+  int2 info{0, 0};
+  uint64_t sid = 0;
   if (r < total) {
     int p = 0;
     int64_t start = 0;
@@ -786,17 +838,18 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
       start = io.seg[p];
     }
     proc = p;
-    const int64_t oidx = (int64_t)p * io.req_cap + (r - start);
+    oidx = (int64_t)p * io.req_cap + (r - start);
     slot = io.req_slot[oidx];
     const double E = io.req_E[oidx];
-    const int pid = S.pid[slot];
-    const int shower = S.shower[slot];
+    info = io.par.info[oidx];
+    const int pid = info_pid(info);
+    const int shower = info.y;
     const int32_t tr = io.trials_out[oidx];
     if (tr <= 0) {
       if (shower_energy) unsafeAtomicAdd(shower_energy + 4 * shower + 1, E);
     } else {
       const float* xs = io.x_out + 4 * oidx;
-      const uint64_t sid = S.sid[slot];
+      sid = io.par.sid[oidx];
       const double az = u01d(draw_block(seed, sid, 0, PURPOSE_INTERACT).y);
       const double m = pid_mass(pid);
       V4 a, b;
@@ -809,7 +862,7 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
         default: kin_ee(E, (double)xs[0], az, a, b); break;  // Moller, Bhabha
       }
       child_pids(p, pid, &cpid[0], &cpid[1]);
-      const float4 pf = S.pf[slot];
+      const float4 pf = io.par.pf[oidx];
       double nx = pf.y, ny = pf.z, nz = pf.w;
       const double pn = sqrt(nx * nx + ny * ny + nz * nz);
       if (pn > 0.0) {
@@ -846,25 +899,22 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
   base = shfl_u64(base, 63);
   unsigned long long dst = base + incl - nk;
   if (nk) {
-    const int32_t pmeta = S.meta[slot];
-    const int gen = (pmeta & 0xFFFF) + 1;
-    const float4 rf = S.rf[slot];
-    const float4 r0 = S.r0[slot];
-    const uint64_t psid = S.sid[slot];
-    const int shower = S.shower[slot];
+    const int gen = info_gen(info) + 1;
+    const float4 pos = io.par.pos[oidx];
+    const int shower = info.y;
     for (int k = 0; k < 2; ++k) {
       if (!keep[k]) continue;
       if ((int64_t)dst < S.capacity) {
         const int64_t c = (int64_t)dst;
         S.p0[c] = make_float4((float)lab[k].e, (float)lab[k].x, (float)lab[k].y, (float)lab[k].z);
         S.pf[c] = S.p0[c];
-        S.r0[c] = make_float4(rf.x, rf.y, rf.z, r0.w);
-        S.rf[c] = make_float4(rf.x, rf.y, rf.z, 0.0f);
+        S.r0[c] = make_float4(pos.x, pos.y, pos.z, pos.w);
+        S.rf[c] = make_float4(pos.x, pos.y, pos.z, 0.0f);
         S.pid[c] = cpid[k];
         S.parent[c] = (int32_t)slot;
         S.meta[c] = (gen & 0xFFFF) | ((proc + 1) << 16) | (k << 24);
         S.shower[c] = shower;
-        S.sid[c] = derive_sid(seed, psid, (uint32_t)k, PURPOSE_CHILD);
+        S.sid[c] = derive_sid(seed, sid, (uint32_t)k, PURPOSE_CHILD);
       } else if (stats) {
         atomicAdd(stats + ST_OVERFLOW, 1ull);
       }
@@ -918,6 +968,7 @@ struct PetiteCtx {
   float* res_w = nullptr;
   int64_t req_cap = 0;
   unsigned int* req_count = nullptr;
+  ParentRecs par{};  // [kNprocSM][req_cap], shower pipeline only
   unsigned long long* counters = nullptr;  // C_N entries, layout CounterIdx
   bool timing = false;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -994,6 +1045,10 @@ static int ensure_requests(PetiteCtx* c, int64_t cap) {
     c->req_slot = nullptr;
     c->req_cap = 0;
   }
+  if (c->par.sid) {
+    for (void* p : {(void*)c->par.sid, (void*)c->par.pf, (void*)c->par.pos, (void*)c->par.info}) (void)hipFree(p);
+    c->par = ParentRecs{};
+  }
   const int64_t n = cap * PETITE_NPROC;
   HIP_TRY(hipMalloc(&c->req_slot, n * sizeof(int32_t)));
   HIP_TRY(hipMalloc(&c->req_E, n * sizeof(double)));
@@ -1001,6 +1056,17 @@ static int ensure_requests(PetiteCtx* c, int64_t cap) {
   HIP_TRY(hipMalloc(&c->res_trials, n * sizeof(int32_t)));
   HIP_TRY(hipMalloc(&c->res_w, n * sizeof(float)));
   c->req_cap = cap;
+  return PETITE_OK;
+}
+
+// the parents' records beside the requests of the SM processes
+static int ensure_parent_recs(PetiteCtx* c) {
+  if (c->par.sid) return PETITE_OK;
+  const int64_t n = c->req_cap * kNprocSM;
+  HIP_TRY(hipMalloc(&c->par.sid, n * sizeof(uint64_t)));
+  HIP_TRY(hipMalloc(&c->par.pf, n * sizeof(float4)));
+  HIP_TRY(hipMalloc(&c->par.pos, n * sizeof(float4)));
+  HIP_TRY(hipMalloc(&c->par.info, n * sizeof(int2)));
   return PETITE_OK;
 }
 
@@ -1060,6 +1126,20 @@ static int ensure_draw_streams(PetiteCtx* c) {
 
 // Device-planned draws of the shower pipeline: every SM process with a grid
 // gets a whole-GPU launch on its stream; blocks the count does not need leave.
+// Blocks of a device-planned draw launch of process p, the processes running
+// side by side: Brem and pair production hold most of a shower's draws (and
+// the longest accept/reject loops), so they get the most of the GPU; the
+// others finish early, after which Brem runs alone at full occupancy
+// (4 blocks of 4 waves per CU).  Measured on MI355X against n_cu * 2 blocks
+// for every process: draws 10 % shorter.
+static int draw_blocks(const PetiteCtx* c, int p) {
+  switch (p) {
+    case BREM: return c->n_cu * 4;
+    case PAIRPROD: return c->n_cu * 2;
+    default: return c->n_cu;
+  }
+}
+
 static int launch_draws_device(PetiteCtx* c, const DrawLaunch& io, uint64_t seed, hipStream_t s, bool muons) {
   int rc = ensure_draw_streams(c);
   if (rc) return rc;
@@ -1068,7 +1148,9 @@ static int launch_draws_device(PetiteCtx* c, const DrawLaunch& io, uint64_t seed
   for (int p = 0; p < 8; ++p) {
     if (!used(p)) continue;
     HIP_TRY(hipStreamWaitEvent(c->dstream[p], c->dfork, 0));
-    hipLaunchKernelGGL(kDrawKernels[p], dim3((unsigned)io.full_blocks), dim3(256), 0, c->dstream[p], c->dev, io, seed);
+    DrawLaunch iop = io;
+    iop.full_blocks = draw_blocks(c, p);
+    hipLaunchKernelGGL(kDrawKernels[p], dim3((unsigned)iop.full_blocks), dim3(256), 0, c->dstream[p], c->dev, iop, seed);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(c->djoin[p], c->dstream[p]));
   }
@@ -1183,7 +1265,8 @@ void petite_ctx_destroy(PetiteCtx* c) {
   }
   for (void* p : {(void*)c->d64, (void*)c->d32, (void*)c->dmaxf, (void*)c->req_slot, (void*)c->req_E, (void*)c->res_x,
                   (void*)c->res_trials, (void*)c->res_w, (void*)c->req_count, (void*)c->counters, (void*)c->ddesc,
-                  (void*)c->dsig_lds, (void*)c->dark_blob, (void*)c->dyield, (void*)c->yield_blob, (void*)c->yacc, (void*)c->ytab_dev})
+                  (void*)c->dsig_lds, (void*)c->dark_blob, (void*)c->dyield, (void*)c->yield_blob, (void*)c->yacc, (void*)c->ytab_dev,
+                  (void*)c->par.sid, (void*)c->par.pf, (void*)c->par.pos, (void*)c->par.info})
     if (p) (void)hipFree(p);
   free_entries(c);
   delete c;
@@ -1289,6 +1372,8 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   const int64_t bound = std::min<int64_t>(store->capacity, kSweepBound);
   int rc = ensure_requests(c, bound);
   if (rc) return rc;
+  rc = ensure_parent_recs(c);
+  if (rc) return rc;
   if (!c->drange) HIP_TRY(hipMalloc(&c->drange, 4 * sizeof(int64_t)));
   unsigned long long* ctr = c->counters;
   HIP_TRY(hipMemsetAsync(ctr, 0, C_N * sizeof(unsigned long long), s));
@@ -1307,6 +1392,7 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   io.req_slot = c->req_slot;
   io.req_E = c->req_E;
   io.sid_src = store->sid;
+  io.req_sid = c->par.sid;
   io.req_cap = c->req_cap;
   io.explicit_out = 0;
   io.x_out = c->res_x;
@@ -1314,7 +1400,7 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   io.w_out = c->res_w;
   io.counters = ctr;
   io.dcount = c->req_count;
-  io.full_blocks = c->n_cu * 2;  // per process: the processes run side by side
+  io.full_blocks = 0;  // per process, set by launch_draws_device
   EmitLaunch em{};
   em.req_slot = c->req_slot;
   em.req_E = c->req_E;
@@ -1322,6 +1408,7 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   em.x_out = c->res_x;
   em.trials_out = c->res_trials;
   em.dcount = c->req_count;
+  em.par = c->par;
   const unsigned tr_blocks = (unsigned)(c->n_cu * 5);  // 20 waves per CU
   const unsigned em_blocks = (unsigned)(c->n_cu * 8);
   const size_t lds = (size_t)c->dev.lds_doubles * sizeof(double);
@@ -1342,7 +1429,7 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 0], s));
       hipLaunchKernelGGL(k_transport, dim3(tr_blocks), dim3(256), lds, s, c->dev, S, (int64_t)0, (int64_t)0, seed, R,
-                         shower_energy, ctr, c->dsig_lds, (const int64_t*)c->drange);
+                         c->par, shower_energy, ctr, c->dsig_lds, (const int64_t*)c->drange);
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 1], s));
       rc = launch_draws_device(c, io, seed, s, muons);
