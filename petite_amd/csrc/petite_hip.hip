@@ -457,10 +457,14 @@ constexpr int kNprocSM = 8;
 __device__ inline int info_pid(int2 v) { return (int)(int16_t)(v.x & 0xFFFF); }
 __device__ inline int info_gen(int2 v) { return (v.x >> 16) & 0xFFFF; }
 
-// n*sigma tables in LDS.  Header per process p (4 doubles): ln E0,
-// (n-1)/ln(E_last/E0), offset of the energies, n; the energies, the values and
-// the interval slopes (y[i+1]-y[i])/(x[i+1]-x[i]) follow, so a lookup is
-// slope * (E - x[lo]) + y[lo], the arithmetic of scipy's interp1d.
+// n*sigma tables in LDS.  Header per process p (kLdsHeader doubles): ln E0,
+// (n-1)/ln(E_last/E0), offset of the interval records, n, E_first, E_last and
+// the value of a one-point table.  Interval i is one 32-byte record
+// {x_i, x_i+1, y_i, (y_i+1 - y_i)/(x_i+1 - x_i)}, so a lookup is one header
+// read and one record read; the value is slope * (E - x_i) + y_i, the
+// arithmetic of scipy's interp1d (0 outside the table).
+constexpr int kLdsHeader = 8;
+
 // Channels of a charged species' n*sigma sum, in the order of its terms
 // (shower.py:355-366), 4 bits each, 0xF = none; the sum is 0 without channels.
 __device__ inline uint32_t sum_channels(int pid) {
@@ -476,24 +480,42 @@ struct LdsSigma {
   const double* base;
   // lf = logf(E), shared by the lookups of one energy (geometric-grid guess)
   __device__ inline double nsigma(int p, double E, float lf) const {
-    const double* h = base + 4 * p;
+    const double* h = base + kLdsHeader * p;
     const int n = (int)h[3];
     if (n <= 0) return 0.0;
-    const double* xe = base + (int)h[2];
-    const double* ye = xe + n;
-    if (!(E >= xe[0]) || E > xe[n - 1]) return 0.0;
-    if (n == 1) return ye[0];
+    if (!(E >= h[4]) || E > h[5]) return 0.0;
+    if (n == 1) return h[6];
+    const double* iv = base + (int)h[2];
     int lo;
     const double inv_dlog = h[1];
     if (inv_dlog > 0.0) {
       lo = (int)((lf - (float)h[0]) * (float)inv_dlog);
       lo = lo < 0 ? 0 : (lo > n - 2 ? n - 2 : lo);
-      while (lo > 0 && xe[lo] > E) --lo;
-      while (lo < n - 2 && xe[lo + 1] <= E) ++lo;
     } else {
-      lo = grid_interval(xe, n, 0.0, 0.0, E);
+      lo = 0;
+      int hi = n - 1;
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (iv[4 * mid] <= E) lo = mid; else hi = mid;
+      }
     }
-    return ye[n + lo] * (E - xe[lo]) + ye[lo];
+    // the largest lo <= n-2 with x_lo <= E (the guess is off by one at most
+    // near a node, where float32 log and the float64 grid disagree)
+    const double* r = iv + 4 * lo;
+    double xl = r[0], xh = r[1];
+    while (lo > 0 && xl > E) {
+      --lo;
+      r -= 4;
+      xl = r[0];
+      xh = r[1];
+    }
+    while (lo < n - 2 && xh <= E) {
+      ++lo;
+      r += 4;
+      xl = r[0];
+      xh = r[1];
+    }
+    return r[3] * (E - xl) + r[2];
   }
   // _NSigmaElectron/Photon/Positron/Muon (shower.py:355-366), terms in their order
   __device__ inline double nsigma_sum(int pid, double E) const {
@@ -648,7 +670,7 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
           } else if (pid == 22) {
             // one exponential flight, no MCS (shower.py:538-553)
             const double mfp = L.mfp(22, E);
-            const double dist = -mfp * (double)log1pf(-u01f(draw_block(seed, sid, 0, PURPOSE_STEP).x));
+            const double dist = -mfp * (double)log1p_f32(-u01f(draw_block(seed, sid, 0, PURPOSE_STEP).x));
             rx += (float)(nx * dist);
             ry += (float)(ny * dist);
             rz += (float)(nz * dist);
@@ -766,7 +788,7 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
         } else {
           // dz / mfp = dz * S / 0.01
           const float r2 = S > 0.0 ? (float)(dz * S * (1.0 / kCmToM)) : (float)(dz * 1.0e-12);
-          dist = -mfp * (double)log1pf(expm1f(-r2) * (float)distC);
+          dist = -mfp * (double)log1p_f32(expm1f(-r2) * (float)distC);
         }
         w_r = u.y;
         w_a = u.z;
@@ -1131,12 +1153,12 @@ static int ensure_draw_streams(PetiteCtx* c) {
 // the longest accept/reject loops), so they get the most of the GPU; the
 // others finish early, after which Brem runs alone at full occupancy
 // (4 blocks of 4 waves per CU).  Measured on MI355X against n_cu * 2 blocks
-// for every process: draws 10 % shorter.
+// for every process: draws 13 % shorter.
 static int draw_blocks(const PetiteCtx* c, int p) {
   switch (p) {
     case BREM: return c->n_cu * 4;
     case PAIRPROD: return c->n_cu * 2;
-    default: return c->n_cu;
+    default: return c->n_cu / 2;
   }
 }
 
@@ -1223,19 +1245,26 @@ int petite_ctx_create(const PetiteTableDesc* desc, const double* blob64, int64_t
   T.mcs_cC = (float)(3.34 * (Z * kAlpha) * (Z * kAlpha));
   {
     // n*sigma tables of the transport kernel, laid out for LDS
-    std::vector<double> lds(4 * PETITE_NPROC, 0.0);
+    std::vector<double> lds(kLdsHeader * PETITE_NPROC, 0.0);
     for (int p = 0; p < PETITE_NPROC; ++p) {
       const PetiteSigmaDesc& sd = desc->sigma[p];
-      lds[4 * p + 0] = sd.lnE0;
-      lds[4 * p + 1] = sd.inv_dlog;
-      lds[4 * p + 3] = sd.n;
+      double* h = &lds[kLdsHeader * p];
+      h[0] = sd.lnE0;
+      h[1] = sd.inv_dlog;
+      h[3] = sd.n;
       if (sd.n <= 0) continue;
-      lds[4 * p + 2] = (double)lds.size();
-      for (int i = 0; i < sd.n; ++i) lds.push_back(blob64[sd.E_off + i]);
-      for (int i = 0; i < sd.n; ++i) lds.push_back(blob64[sd.v_off + i]);
-      for (int i = 0; i + 1 < sd.n; ++i)
-        lds.push_back((blob64[sd.v_off + i + 1] - blob64[sd.v_off + i]) /
-                      (blob64[sd.E_off + i + 1] - blob64[sd.E_off + i]));
+      const double* x = blob64 + sd.E_off;
+      const double* y = blob64 + sd.v_off;
+      h[4] = x[0];
+      h[5] = x[sd.n - 1];
+      h[6] = y[0];
+      h[2] = (double)lds.size();
+      for (int i = 0; i + 1 < sd.n; ++i) {
+        lds.push_back(x[i]);
+        lds.push_back(x[i + 1]);
+        lds.push_back(y[i]);
+        lds.push_back((y[i + 1] - y[i]) / (x[i + 1] - x[i]));
+      }
     }
     T.lds_doubles = (int32_t)lds.size();
     if (hipMalloc(&c->dsig_lds, lds.size() * sizeof(double)) != hipSuccess ||

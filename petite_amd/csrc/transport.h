@@ -184,6 +184,16 @@ __device__ inline void frame_apply_d(double nx, double ny, double nz, const V4& 
 // Lynch-Dahl Gaussian width (moliere.py:196-219, 265-284) with the material
 // constants folded: chi_c^2 = cA t / (p beta)^2 and
 // v = chi_c^2 / (2 (1-F) chi_a^2) = cA t / (2 (1-F) cB (beta^2 + cC)).
+// ln(1 + y) for y > -1 to a few ulp: Goldberg's correction of the rounded
+// sum u = 1 + y (D. Goldberg, What every computer scientist should know about
+// floating-point arithmetic, ACM Comput. Surv. 23 (1991), Theorem 4), with the
+// hardware logarithm and reciprocal.
+__device__ inline float log1p_f32(float y) {
+  const float u = 1.0f + y;
+  const float d = u - 1.0f;
+  return d == 0.0f ? y : __logf(u) * (y * __builtin_amdgcn_rcpf(d));
+}
+
 __device__ inline float lynch_dahl_theta0(float cA, float cB, float cC, float t_gcm2, float pMeV, float beta) {
   // quotients with the hardware reciprocal (about 1 ulp): the angle is a float32 quantity
   const float F = 0.98f;
@@ -196,7 +206,7 @@ __device__ inline float lynch_dahl_theta0(float cA, float cB, float cC, float t_
     // sum_{n>=1} (-1)^{n+1} v^n / (n (n+1))
     g = v * (0.5f + v * (-1.0f / 6.0f + v * (1.0f / 12.0f + v * (-1.0f / 20.0f + v * (1.0f / 30.0f + v * (-1.0f / 42.0f + v * (1.0f / 56.0f)))))));
   } else {
-    g = (1.0f + v) * log1pf(v) * __builtin_amdgcn_rcpf(v) - 1.0f;
+    g = (1.0f + v) * log1p_f32(v) * __builtin_amdgcn_rcpf(v) - 1.0f;
   }
   return sqrtf(chic2 * g * (1.0f / (1.0f + F * F)));
 }
