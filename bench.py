@@ -32,11 +32,9 @@ SEED = 1234
 MATERIAL = "lead"
 CUTOFF = 0.010
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
-# algorithmic bytes per unit of work: a particle-step reads and
-# writes a 64-byte live record; an emitted particle writes a 96-byte record;
-# a primary is read once (32 bytes)
-B_STEP, B_OUT, B_PRIM = 128, 96, 32
+# algorithmic bytes of one particle-step: it reads and writes a 64-byte live
+# particle record
+B_STEP = 128
 
 
 def measured_traffic(kernel, alg_bytes_per_launch):
@@ -133,7 +131,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): showers are independent, so a multi-GPU "
-              "run is a sharding of primaries with no exchange; see DESIGN.md", file=sys.stderr)
+              "run is a sharding of primaries with no exchange between GPUs", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -150,7 +148,9 @@ def main():
         del b
     torch.cuda.synchronize()
     totals = {"steps": 0, "particles": 0, "draws": 0, "trials": 0}
-    kern = {"transport": [0.0, 0], "draw": [0.0, 0], "emit": [0.0, 0]}
+    # stage times from HIP events on the run's stream: the transport kernel, and
+    # the draws with their emissions (per-process streams, fork to join)
+    kern = {"transport": [0.0, 0], "draw_emit": [0.0, 0]}
     t0 = time.perf_counter()
     for k in range(args.steps):
         b = sh.generate_showers(prim, seed=SEED)
@@ -159,9 +159,10 @@ def main():
         totals["particles"] += st["n_particles"]
         totals["draws"] += st["n_draws"]
         totals["trials"] += st["n_trials"]
-        for key in kern:
-            kern[key][0] += st[f"ms_{key}"]
-            kern[key][1] += st[f"launches_{key}"]
+        kern["transport"][0] += st["ms_transport"]
+        kern["transport"][1] += st["launches_transport"]
+        kern["draw_emit"][0] += st["ms_draw"] + st["ms_emit"]
+        kern["draw_emit"][1] += st["launches_draw"]
         print(f"step {k}: {b.n} particles, {st['n_steps']} particle-steps", file=sys.stderr, flush=True)
         del b
     torch.cuda.synchronize()
@@ -170,29 +171,15 @@ def main():
     steps_per_s = totals["steps"] / dt
     showers_per_s = K * len(prim) / dt
 
-    # dominant kernel and its roofline
-    dom = max(kern, key=lambda k: kern[k][0])
-    t_dom = kern[dom][0] / 1e3
-    if dom == "transport":
-        alg = B_STEP * totals["steps"]
-        ach = alg / t_dom / 1e9
-        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                "traffic": None, "kernel": "k_transport", "per_unit": "128 B per particle-step"}
-    elif dom == "emit":
-        alg = B_OUT * totals["particles"]
-        ach = alg / t_dom / 1e9
-        roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                "traffic": None, "kernel": "k_emit", "per_unit": "96 B per emitted particle"}
-    else:
-        # accept/reject candidates: vector-ALU bound; flops counted per candidate
-        flops = FLOPS_PER_CANDIDATE * totals["trials"]
-        ach = flops / t_dom / 1e12
-        roof = {"bound": "mfma", "achieved": ach, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "k_draw",
-                "per_unit": f"{FLOPS_PER_CANDIDATE} flop per candidate (vector ALU; no MFMA used)"}
-    roof["avg_launch_ms"] = kern[dom][0] / max(kern[dom][1], 1)
-    if roof["bound"] == "hbm":
-        roof["traffic"] = measured_traffic(roof["kernel"], alg / max(kern[dom][1], 1))
+    # roofline of the dominant kernel, k_transport (the largest single kernel in
+    # the rocprofv3 summary; the draw kernels of the six processes run side by side)
+    t_dom = kern["transport"][0] / 1e3
+    alg = B_STEP * totals["steps"]
+    ach = alg / t_dom / 1e9
+    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+            "traffic": None, "kernel": "k_transport", "per_unit": "128 B per particle-step"}
+    roof["avg_launch_ms"] = kern["transport"][0] / max(kern["transport"][1], 1)
+    roof["traffic"] = measured_traffic(roof["kernel"], alg / max(kern["transport"][1], 1))
     kernels = {k: {"ms_total": v[0] / K, "launches": v[1] // K} for k, v in kern.items()}
 
     cpu = None
@@ -252,10 +239,6 @@ def main():
         out["dark_yield_scan"] = yield_out
     print(json.dumps(out))
 
-
-# float operations of one accept/reject candidate, averaged over the mix of
-# processes (map transform + integrand; Philox integer work excluded)
-FLOPS_PER_CANDIDATE = 120
 
 # dark workloads
 DARK_N, DARK_E, DARK_MAT, DARK_MV = 20_000, 10.0, "graphite", 0.001

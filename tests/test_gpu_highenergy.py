@@ -1,5 +1,5 @@
-"""High-energy batch on the 400 GeV table set (SURVEY C5 shape, reduced):
-400 GeV muons and electrons in lead, cutoff 100 MeV, m_V = 10 MeV.  The
+"""High-energy batch on the 400 GeV table set, six primaries: 400 GeV muons
+and electrons in lead, cutoff 100 MeV, m_V = 10 MeV.  The
 400 GeV set lacks the Brem/PairProd/MuonBrem/DarkBrem/DarkMuonBrem maps, so
 the 100 GeV maps stand in (their variables are dimensionless).  Checked on the
 device record: per-shower float64 energy budget, muon processes present,

@@ -1,5 +1,5 @@
-"""Statistical agreement of GPU showers with the oracle (SURVEY C1 shape):
-10 GeV electrons in graphite, cutoff 10 MeV.  The oracle's showers are a
+"""Statistical agreement of GPU showers with the oracle: 10 GeV electrons in
+graphite, cutoff 10 MeV.  The oracle's showers are a
 committed fixture (tests/golden/make_c1_fixture.py, 320 showers, its own
 seed); the GPU runs 2,000 showers from another seed, so the samples are
 independent.  Particles of one shower are correlated, so every observable is
