@@ -453,6 +453,8 @@ __global__ __launch_bounds__(256) void k_dark_emit(DevTables T, PetiteDarkDesc D
 
 // ------------------------------------------------------ m_V-continuous yield --
 constexpr int kYieldBatch = 16;
+constexpr int kYieldWindow = 4;  // trained masses a pass may use (consecutive)
+constexpr double kLn10 = 2.302585092994045684;
 
 struct YieldTables {
   PetiteDarkDesc d[PETITE_MAX_MASSES];
@@ -460,16 +462,56 @@ struct YieldTables {
   const double* b;
 };
 
-// Mass points of one pass: segment j (masses j, j+1), t and dt/dm_V.
+// Mass points of one pass (ascending m_V): segment j0 + j[k] (trained masses
+// j0 + j[k] and the next), t and dt/dm_V; `need` marks which of the window's
+// trained masses j0 .. j0 + kYieldWindow - 1 the segments use.
 struct YieldPoints {
   int32_t n;
+  int32_t j0;
+  uint32_t need;
   int32_t j[kYieldBatch];
   double t[kYieldBatch];
   double dtdm[kYieldBatch];
+  double ln_eps2[kYieldWindow];
 };
 
+// interpolate1d in log10-log10 (dark_shower.py:31-46), returning the log10 of
+// the value: -20 outside the table (the fill 1e-20); l = log10(E).
+__device__ inline double loglog10(const double* lx, const double* ly, int n, double l, double l0, double inv_dl) {
+  if (n <= 0 || l < lx[0] || l > lx[n - 1]) return -20.0;
+  if (l == lx[n - 1]) return ly[n - 1];
+  const int lo = find_interval(lx, n, l, (float)l, l0, inv_dl);
+  const double slope = (ly[lo + 1] - ly[lo]) / (lx[lo + 1] - lx[lo]);
+  return slope * (l - lx[lo]) + ly[lo];
+}
+
+// Weight of one trained mass for the yield, as (w, ln w): GetBSMWeights with
+// that mass's tables (dark_shower.py:595-647).  DarkComp's log-log table gives
+// ln w = ln eps^2 - ln(n sigma_PP + n sigma_Comp) + ln 10 * L directly (w
+// itself, eps^2 10^L / den, is formed only where the linear branch needs it;
+// here w = 1 marks it positive); the other families take the log of the
+// interpolated weight.
+__device__ inline void yield_weight(const DevTables& T, const PetiteDarkDesc& D, const double* b, int pid, int code,
+                                    const WeightArgs& a, double ln_eps2, double ln_den, double& wv, double& lam) {
+  if (code == DARKCOMP) {
+    if (pid != 22 || a.E0 < D.min_comp || a.E0 < T.d->min_calc[2]) {
+      wv = 0.0;
+      lam = 0.0;
+      return;
+    }
+    const double* lx = b + D.comp_off;
+    lam = ln_eps2 - ln_den + kLn10 * loglog10(lx, lx + D.comp_n, D.comp_n, a.l10, D.comp_l0, D.comp_inv_dl);
+    wv = 1.0;
+    return;
+  }
+  wv = bsm_weight(T, D, b, pid, code, a);
+  lam = wv > 0.0 ? log(wv) : 0.0;
+}
+
 // One float64 pass over the record: acc[2k] += w W(m_k), acc[2k+1] += w dW/dm(m_k)
-// (GRAD = false: the yield alone).
+// (GRAD = false: the yield alone).  Between trained masses j and j+1 a
+// particle's weight is interpolated geometrically, W = exp(ln w_j + t ln(w_j+1 /
+// w_j)), where both are positive, linearly otherwise.
 template <bool GRAD>
 __global__ __launch_bounds__(256) void k_dark_yield(DevTables T, const YieldTables* __restrict__ Yp, DevStore S,
                                                     int64_t n, YieldPoints P, double* acc) {
@@ -484,36 +526,42 @@ __global__ __launch_bounds__(256) void k_dark_yield(DevTables T, const YieldTabl
     const double E0 = (double)S.p0[i].x;
     const double w = (double)S.r0[i].w;
     const WeightArgs wa = weight_args(T, pid, E0);
+    const double ln_den = pid == 22 ? log(wa.den) : 0.0;
     for (int a = 0; a < d0.n_active; ++a) {
       const int code = d0.active[a];
       if (code == DARKCOMP ? pid != 22 : family_of(code, pid) < 0) continue;
-      double wj[PETITE_MAX_MASSES], rj[PETITE_MAX_MASSES];
+      double wv[kYieldWindow], lam[kYieldWindow];
 #pragma unroll
-      for (int j = 0; j < PETITE_MAX_MASSES; ++j)
-        wj[j] = j < Y.n_mass ? bsm_weight(T, Y.d[j], Y.b, pid, code, wa) : 0.0;
-      // log ratio of each segment, formed once per particle for all masses
-#pragma unroll
-      for (int j = 0; j + 1 < PETITE_MAX_MASSES; ++j)
-        rj[j] = (j + 1 < Y.n_mass && wj[j] > 0.0 && wj[j + 1] > 0.0) ? log(wj[j + 1] / wj[j]) : 0.0;
-      rj[PETITE_MAX_MASSES - 1] = 0.0;
+      for (int q = 0; q < kYieldWindow; ++q) {
+        wv[q] = lam[q] = 0.0;
+        if (P.need & (1u << q))
+          yield_weight(T, Y.d[P.j0 + q], Y.b, pid, code, wa, P.ln_eps2[q], ln_den, wv[q], lam[q]);
+      }
 #pragma unroll
       for (int k = 0; k < kYieldBatch; ++k) {
         if (k >= P.n) break;
-        const int j = P.j[k];
-        double lo = 0.0, hi = 0.0, r = 0.0;
-#pragma unroll
-        for (int jj = 0; jj < PETITE_MAX_MASSES; ++jj) {
-          lo = jj == j ? wj[jj] : lo;
-          hi = jj == j + 1 ? wj[jj] : hi;
-          r = jj == j ? rj[jj] : r;
+        double lo = 0.0, hi = 0.0, llo = 0.0, lhi = 0.0;
+        switch (P.j[k]) {  // uniform over the grid
+          case 0: lo = wv[0]; llo = lam[0]; hi = wv[1]; lhi = lam[1]; break;
+          case 1: lo = wv[1]; llo = lam[1]; hi = wv[2]; lhi = lam[2]; break;
+          case 2: lo = wv[2]; llo = lam[2]; hi = wv[3]; lhi = lam[3]; break;
+          default: lo = wv[3]; llo = lam[3]; hi = lo; lhi = llo; break;
         }
-        if (j + 1 >= Y.n_mass) hi = lo;
+        if (P.j0 + P.j[k] + 1 >= Y.n_mass) {
+          hi = lo;
+          lhi = llo;
+        }
         const double t = P.t[k];
         double W, dW;
         if (lo > 0.0 && hi > 0.0) {
-          W = lo * exp(t * r);
+          const double r = lhi - llo;
+          W = exp(llo + t * r);
           dW = GRAD ? W * r * P.dtdm[k] : 0.0;
         } else {
+          if (code == DARKCOMP) {
+            lo = lo > 0.0 ? exp(llo) : 0.0;
+            hi = hi > 0.0 ? exp(lhi) : 0.0;
+          }
           W = lo + t * (hi - lo);
           dW = GRAD ? (hi - lo) * P.dtdm[k] : 0.0;
         }

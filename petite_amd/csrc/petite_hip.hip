@@ -1702,30 +1702,51 @@ int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t
              sm->shower, sm->sid, sm->capacity};
   const int nm = c->n_mass;
   const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n_sm + 255) / 256, (int64_t)c->n_cu * 8));
-  for (int k0 = 0; k0 < n_points; k0 += dark::kYieldBatch) {
-    dark::YieldPoints P{};
-    P.n = std::min(dark::kYieldBatch, n_points - k0);
-    for (int k = 0; k < P.n; ++k) {
-      const double m = mV[k0 + k];
-      int j = 0;
-      while (j + 1 < nm - 1 && c->masses[j + 1] <= m) ++j;
-      double t = 0.0, dtdm = 0.0;
-      if (nm > 1) {
-        const double dl = std::log(c->masses[j + 1] / c->masses[j]);
-        t = std::log(m / c->masses[j]) / dl;
-        dtdm = 1.0 / (m * dl);
-        if (t <= 0.0) {
-          t = 0.0;
-          if (m < c->masses[0]) dtdm = 0.0;
-        }
-        if (t >= 1.0) {
-          t = 1.0;
-          if (m > c->masses[nm - 1]) dtdm = 0.0;
-        }
+  // passes over the points in ascending m_V, so that a pass needs few trained masses
+  std::vector<int> order(n_points);
+  for (int k = 0; k < n_points; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return mV[a] < mV[b]; });
+  // segment and interpolation parameter of every point, ascending m_V
+  std::vector<int> seg(n_points);
+  std::vector<double> tt(n_points), dd(n_points);
+  for (int q = 0; q < n_points; ++q) {
+    const double m = mV[order[q]];
+    int j = 0;
+    while (j + 1 < nm - 1 && c->masses[j + 1] <= m) ++j;
+    double t = 0.0, dtdm = 0.0;
+    if (nm > 1) {
+      const double dl = std::log(c->masses[j + 1] / c->masses[j]);
+      t = std::log(m / c->masses[j]) / dl;
+      dtdm = 1.0 / (m * dl);
+      if (t <= 0.0) {
+        t = 0.0;
+        if (m < c->masses[0]) dtdm = 0.0;
       }
+      if (t >= 1.0) {
+        t = 1.0;
+        if (m > c->masses[nm - 1]) dtdm = 0.0;
+      }
+    }
+    seg[q] = j;
+    tt[q] = t;
+    dd[q] = dtdm;
+  }
+  // passes of at most kYieldBatch points whose segments fit one window of
+  // kYieldWindow consecutive trained masses
+  for (int k0 = 0; k0 < n_points;) {
+    dark::YieldPoints P{};
+    P.j0 = seg[k0];
+    P.n = 0;
+    while (P.n < dark::kYieldBatch && k0 + P.n < n_points && seg[k0 + P.n] + 1 < P.j0 + dark::kYieldWindow) ++P.n;
+    for (int q = 0; q < dark::kYieldWindow; ++q)
+      P.ln_eps2[q] = P.j0 + q < nm ? std::log(c->ytab.d[P.j0 + q].eps2) : 0.0;
+    for (int k = 0; k < P.n; ++k) {
+      const int j = seg[k0 + k] - P.j0;
       P.j[k] = j;
-      P.t[k] = t;
-      P.dtdm[k] = dtdm;
+      P.t[k] = tt[k0 + k];
+      P.dtdm[k] = dd[k0 + k];
+      P.need |= 1u << j;
+      if (P.j0 + j + 1 < nm) P.need |= 1u << (j + 1);
     }
     HIP_TRY(hipMemsetAsync(c->yacc, 0, 2 * dark::kYieldBatch * sizeof(double), s));
     if (n_sm > 0) {
@@ -1741,9 +1762,10 @@ int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t
     HIP_TRY(hipMemcpyAsync(h, c->yacc, sizeof(h), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     for (int k = 0; k < P.n; ++k) {
-      yield_host[k0 + k] = h[2 * k];
-      if (dyield_host) dyield_host[k0 + k] = h[2 * k + 1];
+      yield_host[order[k0 + k]] = h[2 * k];
+      if (dyield_host) dyield_host[order[k0 + k]] = h[2 * k + 1];
     }
+    k0 += P.n;
   }
   return PETITE_OK;
 }
