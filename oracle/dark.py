@@ -277,7 +277,7 @@ class OracleDark:
 #   INTERACT block 0: x drate bin, y MCS radius, z MCS azimuth (sign: bit 0 of z), w kinematic azimuth
 #   INTERACT block 1: x bound-electron cos(theta_e) (2u - 1), y rotation azimuth of the bound-electron frame
 #   BOUND_E block k:  (x, y) and (z, w) accept/reject pairs of the bound-electron momentum
-#   VEGAS_A / VEGAS_B: the accept/reject candidates of draw_dark_sample
+#   VEGAS_A: the accept/reject candidates of draw_dark_sample
 W_FAMILY = {("DarkBrem", 11): "brem_elec", ("DarkBrem", -11): "brem_positron", ("DarkAnn", -11): "annihilation",
             ("DarkMuonBrem", 13): "muon_brem", ("DarkMuonBrem", -13): "muon_brem"}
 PE_MAX_PAIRS = 1 << 16

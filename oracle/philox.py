@@ -24,8 +24,7 @@ MASK32 = np.uint64(0xFFFFFFFF)
 PURPOSE_STEP = 0       # substep k: x hard test, y -> U(6,20), z MCS radius, w MCS azimuth (sign: bit 0 of w)
 PURPOSE_CLOSE = 2      # closing increment: x fraction, y MCS radius, z MCS azimuth (sign: bit 0 of z)
 PURPOSE_INTERACT = 3   # index 0: u0 process choice, u1 kinematic azimuth, u2/u3 extra
-PURPOSE_VEGAS_A = 4    # candidate j: y0..y3 (ndim<=3: u3 is the accept uniform)
-PURPOSE_VEGAS_B = 5    # candidate j (ndim 4): u0 accept uniform
+PURPOSE_VEGAS_A = 4    # candidate j: y0..y3 (ndim<=3: u3 is the accept uniform; ndim 4: low bits, see vegas)
 PURPOSE_CHILD = 6      # index k: child stream id (words 0,1)
 PURPOSE_DARK = 7       # index = dark process code: dark stream id (words 0,1)
 PURPOSE_BOUND_E = 8    # index j: bound-electron momentum accept/reject pairs

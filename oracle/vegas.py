@@ -13,7 +13,7 @@ Accept/reject follows draw_sample, shower.py:401-465: candidates j = 0, 1, ...
 in order; accept the first with max_F * U_j < J_j f(x_j), where f is
 evaluated at the true energy E and the map is the one at
 LU_Key = argmin|E_i - E| + 1.  Uniforms come from the Philox streams of
-``oracle.philox`` (PURPOSE_VEGAS_A/B).
+``oracle.philox`` (PURPOSE_VEGAS_A, one block per candidate).
 
 Parity of the map: "parity unpinned" at the bit level for vegas itself (not
 installed); pinned through the sigma-table integral identity and the
@@ -48,12 +48,17 @@ def map_points(grid, ie, y):
 
 
 def candidate_uniforms(seed, sid, j, ndim):
-    """(y (n, ndim), u_accept (n,)) for candidate indices j (array)."""
-    a = philox.block(seed, sid, j, philox.PURPOSE_VEGAS_A)
+    """(y (n, ndim), u_accept (n,)) for candidate indices j (array).  One
+    Philox block per candidate: y from the high 23 bits of its words; with
+    four dimensions the accept uniform is made of the unused low bits (9 of
+    word 0, 9 of word 1, 5 of word 2), otherwise it is word 3."""
+    w = philox.words(seed, sid, j, philox.PURPOSE_VEGAS_A)
+    a = philox.to_uniform(w)
     if ndim <= 3:
         return a[..., :ndim], a[..., 3]
-    b = philox.block(seed, sid, j, philox.PURPOSE_VEGAS_B)
-    return a[..., :4], b[..., 0]
+    m9, m5 = np.uint32(0x1FF), np.uint32(0x1F)
+    low = (w[..., 0] & m9) | ((w[..., 1] & m9) << np.uint32(9)) | ((w[..., 2] & m5) << np.uint32(18))
+    return a[..., :4], philox.to_uniform(low << np.uint32(9))
 
 
 def scan_maxf(tables, process, ie, ev, n=MAXF_SAMPLES, seed=MAXF_SEED, ndim=None):

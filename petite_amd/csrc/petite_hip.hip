@@ -182,7 +182,9 @@ __device__ inline double candidate_w(const DevTables& T, double E, const float* 
   const U4 a = draw_block(seed, sid, j, PURPOSE_VEGAS_A);
   const uint32_t rr[4] = {a.x, a.y, a.z, a.w};
   if (ND == 4) {
-    ua = u01d(draw_block(seed, sid, j, PURPOSE_VEGAS_B).x);
+    // the map takes the high 23 bits of the four words; the accept uniform is
+    // made of their low bits (9 of x, 9 of y, 5 of z), independent of them
+    ua = u01d(((a.x & 0x1FFu) | ((a.y & 0x1FFu) << 9) | ((a.z & 0x1Fu) << 18)) << 9);
   } else {
     ua = u01d(a.w);
   }
