@@ -830,11 +830,12 @@ struct EmitLaunch {
 __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunch io, uint64_t seed,
                                               unsigned long long* store_count, double* shower_energy,
                                               unsigned long long* stats) {
-  int64_t total = io.total;
-  if (io.dcount) {
-    total = 0;
-    for (int q = 0; q < PETITE_NPROC; ++q) total += io.dcount[q];
-  }
+  // process boundaries of the request list, read once (uniform)
+  int64_t seg[PETITE_NPROC + 1];
+  seg[0] = 0;
+#pragma unroll
+  for (int q = 0; q < PETITE_NPROC; ++q) seg[q + 1] = io.dcount ? seg[q] + (int64_t)io.dcount[q] : io.seg[q + 1];
+  const int64_t total = io.dcount ? seg[PETITE_NPROC] : io.total;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x; r0 < total; r0 += stride) {
   const int64_t r = r0 + threadIdx.x;
@@ -849,18 +850,11 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
   uint64_t sid = 0;
   if (r < total) {
     int p = 0;
+#pragma unroll
+    for (int q = 1; q < PETITE_NPROC; ++q) p += r >= seg[q] ? 1 : 0;
     int64_t start = 0;
-    if (io.dcount) {
-      int64_t acc = io.dcount[0];
-      while (r >= acc) {
-        start = acc;
-        ++p;
-        acc += io.dcount[p];
-      }
-    } else {
-      while (r >= io.seg[p + 1]) ++p;
-      start = io.seg[p];
-    }
+#pragma unroll
+    for (int q = 0; q < PETITE_NPROC; ++q) start = q == p ? seg[q] : start;
     proc = p;
     oidx = (int64_t)p * io.req_cap + (r - start);
     slot = io.req_slot[oidx];
@@ -930,10 +924,10 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
       if (!keep[k]) continue;
       if ((int64_t)dst < S.capacity) {
         const int64_t c = (int64_t)dst;
+        // pf and rf are left to the transport, which writes them for every
+        // stored particle when it finishes it
         S.p0[c] = make_float4((float)lab[k].e, (float)lab[k].x, (float)lab[k].y, (float)lab[k].z);
-        S.pf[c] = S.p0[c];
         S.r0[c] = make_float4(pos.x, pos.y, pos.z, pos.w);
-        S.rf[c] = make_float4(pos.x, pos.y, pos.z, 0.0f);
         S.pid[c] = cpid[k];
         S.parent[c] = (int32_t)slot;
         S.meta[c] = (gen & 0xFFFF) | ((proc + 1) << 16) | (k << 24);
