@@ -599,8 +599,8 @@ __device__ inline void charged_advance(const DevTables& T, double dist, double m
 // choice, draw requests) run for the whole wave at once, so they wait until
 // kBatchLanes lanes need them (or no lane is stepping): each costs about as
 // much as several substeps (global loads, atomics), and measured on MI355X
-// 32 of 64 lanes gives the shortest sweeps (16: 1.4x, 56: 1.2x longer).
-constexpr int kBatchLanes = 32;
+// 44 of 64 lanes gives the shortest sweeps (32 and 56: 1.13x longer).
+constexpr int kBatchLanes = 44;
 // range (non-null): the sweep's first slot and count, set on the device by
 // k_sweep_setup; blocks beyond what the count needs leave at once.
 __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int64_t first, int64_t n, uint64_t seed,
