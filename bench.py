@@ -129,6 +129,8 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
+    if args.steps < 1:
+        sys.exit("bench.py: --steps must be at least 1")
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): showers are independent, so a multi-GPU "
               "run is a sharding of primaries with no exchange between GPUs", file=sys.stderr)
@@ -164,7 +166,8 @@ def main():
         kern["draw_emit"][0] += st["ms_draw"] + st["ms_emit"]
         kern["draw_emit"][1] += st["launches_draw"]
         print(f"step {k}: {b.n} particles, {st['n_steps']} particle-steps", file=sys.stderr, flush=True)
-        del b
+        if k + 1 < args.steps:
+            del b
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     K = args.steps
@@ -186,6 +189,10 @@ def main():
     if not args.no_cpu_baseline:
         maxf = {(P, ie): float(v) for P, arr in ctx.maxf.items() for ie, v in enumerate(arr)}
         cpu = cpu_baseline(prim, maxf, args.cpu_budget)
+
+    det_out = bench_detector(b, cpu=not args.no_cpu_baseline)
+    print(f"detector: {det_out['records']} records, {det_out['ms']:.3f} ms", file=sys.stderr, flush=True)
+    del b
 
     dark_out = yield_out = None
     if args.dark_forward:
@@ -233,11 +240,53 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu_showers"] = showers_per_s / cpu["showers_per_s"]
+    out["detector_cut"] = det_out
     if dark_out:
         out["dark_forward"] = dark_out
     if yield_out:
         out["dark_yield_scan"] = yield_out
     print(json.dumps(out))
+
+
+def bench_detector(batch, reps=10, cpu=True):
+    """detector_cut (shower.py:825-865) over the whole SM record of the last
+    step: three planes, an annulus and an energy window; HIP events around
+    `reps` launches on the current stream.  40 algorithmic bytes per record
+    (p0 and r0 read, the 64-bit pass mask written)."""
+    import torch
+
+    from petite_amd.detector import cut_masks
+    planes, radius, inner, ecut = [1.0, 2.0, 5.0], 0.5, 0.01, (0.1, 1000.0)
+    p, r = batch.p0, batch.r0
+    n = batch.n
+    cut_masks(p, r, planes, radius, ecut, inner)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        mask, wsum = cut_masks(p, r, planes, radius, ecut, inner)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    ach = 40.0 * n / (ms / 1e3) / 1e9
+    out = {"workload": f"the {n}-record SM record of the value workload; planes z = {planes} m, radius {radius} m, "
+                       f"inner {inner} m, energy window {ecut} GeV",
+           "records": n, "ms": ms, "records_per_s": n / (ms / 1e3),
+           "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": "k_detector_cut",
+                        "per_unit": "40 B per record"},
+           "passing_weight": wsum.cpu().tolist()}
+    if cpu:
+        from oracle import detector as od
+        m = min(n, 4_000_000)
+        pp = p[:m].double().cpu().numpy()
+        rr = r[:m].double().cpu().numpy()
+        t0 = time.perf_counter()
+        od.pass_matrix(pp, rr, planes, radius, inner, ecut)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": m / dt, "unit": "records/s", "cores": 1, "kind": "port",
+                               "sample": f"first {m} records, {dt:.2f} s (oracle numpy float64)"}
+    return out
 
 
 # dark workloads

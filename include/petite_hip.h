@@ -254,6 +254,25 @@ int petite_yield_setup(PetiteCtx* ctx, int32_t n_mass, const double* masses, con
 int petite_dark_yield(PetiteCtx* ctx, const PetiteStore* sm, int64_t n_sm, int32_t n_points, const double* mV_host,
                       double* yield_host, double* dyield_dmV_host, PetiteStream stream);
 
+/* Detector acceptance of straight-line extrapolations: replaces
+ * transverse_position and detector_cut (src/PETITE/shower.py:815-865).
+ * p, r: n records of float4 (E, px, py, pz) and (x, y, z, weight) on the
+ * device -- the production momentum and point (a PetiteStore's p0/r0, or
+ * PetiteDarkOut's p/r).  For detector plane d at z = det_z[d] (host array,
+ * 1 <= n_det <= PETITE_MAX_DETECTORS):
+ *   T = (z_d - z0) / pz,  (x, y) = (x0 + T px, y0 + T py),  r_T = |(x, y)|,
+ * and the record passes when inner_radius < r_T < radius and, if has_ecut,
+ * emin < E < emax.  Arithmetic in float64 without contraction, like numpy.
+ * mask (device, n uint64, may be NULL): bit d = passes detector d.
+ * wsum (device, n_det + 1 float64, may be NULL): wsum[d] = sum of the weights
+ * passing detector d, wsum[n_det] = sum of the weights inside the energy
+ * window (the denominator of the reference's "Efficiency"); the call zeroes
+ * it first.  No context: the kernel needs no tables. */
+#define PETITE_MAX_DETECTORS 64
+int petite_detector_cut(const float* p, const float* r, int64_t n, const double* det_z_host, int32_t n_det,
+                        double radius, double inner_radius, int32_t has_ecut, double emin, double emax,
+                        uint64_t* mask, double* wsum, PetiteStream stream);
+
 #ifdef __cplusplus
 }
 #endif

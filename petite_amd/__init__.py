@@ -10,6 +10,8 @@ from .particle import Particle, default_ids  # noqa: F401
 from .shower import Shower  # noqa: F401
 from .batch import ParticleBatch, PRIMARY_DTYPE  # noqa: F401
 from .dark_shower import DarkBatch, DarkShower  # noqa: F401
+from .detector import detector_cut, transverse_position  # noqa: F401
 
 __all__ = ["Shower", "DarkShower", "DarkBatch", "Particle", "ParticleBatch", "PRIMARY_DTYPE", "process_code", "process_PIDS",
-           "dimensionalities", "mass_dict", "target_information", "default_ids"]
+           "dimensionalities", "mass_dict", "target_information", "default_ids", "detector_cut",
+           "transverse_position"]

@@ -19,7 +19,7 @@ PETITE_ERR_CAPACITY = 4
 
 EXPORTED = ["petite_ctx_create", "petite_ctx_destroy", "petite_last_error", "petite_maxf_scan", "petite_set_maxf",
             "petite_vegas_draw", "petite_eval_points", "petite_run_showers", "petite_set_timing", "petite_dark_setup",
-            "petite_dark_reweight", "petite_yield_setup", "petite_dark_yield"]
+            "petite_dark_reweight", "petite_yield_setup", "petite_dark_yield", "petite_detector_cut"]
 MAX_MASSES = 8
 DARK_FAMILIES = ["brem_elec", "brem_positron", "muon_brem", "annihilation"]
 DARK_DRATE_BINS = 10
@@ -137,6 +137,9 @@ def load():
     lib.petite_yield_setup.restype = C.c_int
     lib.petite_dark_yield.argtypes = [vp, C.POINTER(Store), i64, i32, vp, vp, vp, vp]
     lib.petite_dark_yield.restype = C.c_int
+    f64 = C.c_double
+    lib.petite_detector_cut.argtypes = [vp, vp, i64, vp, i32, f64, f64, i32, f64, f64, vp, vp, vp]
+    lib.petite_detector_cut.restype = C.c_int
     _lib = lib
     return lib
 

@@ -8,11 +8,15 @@ with the reference's ids dictionary, including the arbitrary-precision
 links (the device keeps parent slots, not the doubling IDs, which exceed 64
 bits in deep showers).
 """
+import json
+
 import numpy as np
 import torch
 
 from .constants import mass_dict, process_name
 from .particle import Particle
+
+STORE_KEYS = ("p0", "r0", "pf", "rf", "pid", "parent", "meta", "shower", "sid")
 
 PRIMARY_DTYPE = np.dtype([("E", "f8"), ("px", "f8"), ("py", "f8"), ("pz", "f8"), ("x", "f8"), ("y", "f8"),
                           ("z", "f8"), ("mass", "f8"), ("weight", "f8"), ("pid", "i4"), ("shower", "i4"),
@@ -82,6 +86,31 @@ class ParticleBatch:
         out = {k: self._store[k][: self.n].cpu().numpy() for k in self._store}
         out["sid"] = out["sid"].view(np.uint64)
         return out
+
+    def save(self, path):
+        """Checkpoint of the record as a numpy .npz of plain arrays (no pickled
+        objects): the n rows of every store array, the per-shower energy
+        budget and the run statistics.  ``load`` brings it back to the device;
+        ``DarkShower.generate_dark_shower(ExDir=path)`` reweights it."""
+        arrays = {k: self._store[k][: self.n].cpu().numpy() for k in STORE_KEYS}
+        if self.energy is not None:
+            arrays["energy"] = self.energy.cpu().numpy()
+        if self.primaries is not None:
+            arrays["primaries"] = np.asarray(self.primaries)
+        np.savez(path, petite_record=np.int32(1), n=np.int64(self.n), stats=np.array(json.dumps(self.stats)), **arrays)
+
+    @classmethod
+    def load(cls, path, device="cuda"):
+        """Record written by ``save`` (read with allow_pickle=False)."""
+        with np.load(path, allow_pickle=False) as z:
+            if "petite_record" not in z.files:
+                raise ValueError(f"{path}: not a petite_amd shower record (ParticleBatch.save)")
+            n = int(z["n"])
+            store = {k: torch.from_numpy(np.ascontiguousarray(z[k])).to(device) for k in STORE_KEYS}
+            energy = torch.from_numpy(z["energy"]).to(device) if "energy" in z.files else None
+            prim = z["primaries"] if "primaries" in z.files else None
+            stats = json.loads(str(z["stats"])) if "stats" in z.files else {}
+        return cls(store, n, stats, energy, prim)
 
     def to_particles(self, shower=None):
         """List of Particle objects (all showers, or one), primary first per shower."""

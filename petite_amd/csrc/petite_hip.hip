@@ -817,11 +817,9 @@ struct EmitLaunch {
   const int32_t* req_slot;
   const double* req_E;
   int64_t req_cap;
-  int64_t seg[PETITE_NPROC + 1];  // prefix sums of the per-process counts
-  int64_t total;
   const float* x_out;
   const int32_t* trials_out;
-  const unsigned int* dcount;  // non-null: per-process counts on the device (seg/total unused)
+  const unsigned int* dcount;  // per-process request counts (device)
   ParentRecs par;              // the parents' records, beside the requests
 };
 
@@ -834,8 +832,8 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
   int64_t seg[PETITE_NPROC + 1];
   seg[0] = 0;
 #pragma unroll
-  for (int q = 0; q < PETITE_NPROC; ++q) seg[q + 1] = io.dcount ? seg[q] + (int64_t)io.dcount[q] : io.seg[q + 1];
-  const int64_t total = io.dcount ? seg[PETITE_NPROC] : io.total;
+  for (int q = 0; q < PETITE_NPROC; ++q) seg[q + 1] = seg[q] + (int64_t)io.dcount[q];
+  const int64_t total = seg[PETITE_NPROC];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x; r0 < total; r0 += stride) {
   const int64_t r = r0 + threadIdx.x;
@@ -965,6 +963,106 @@ __global__ void k_sweep_setup(unsigned long long* ctr, int64_t* range, unsigned 
 }
 
 #include "dark.h"
+
+// ------------------------------------------------- detector acceptance --
+// transverse_position and detector_cut (shower.py:815-865): straight-line
+// extrapolation of each record from its production point along its
+// production momentum to the planes z = z_d, an annulus cut on the transverse
+// radius, and the per-plane sums of the passing weights.
+struct DetectorPlanes {
+  int32_t n;
+  int32_t has_ecut;
+  double radius, inner, emin, emax;
+  double z[PETITE_MAX_DETECTORS];
+};
+
+__device__ inline double wave_sum_f64(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += shfl_f64(v, (threadIdx.x & 63) ^ off);
+  return v;
+}
+
+// Pass bits of one record for planes [0, n_planes): T = (z - z0) / pz,
+// r_T = |(x0 + T px, y0 + T py)|, inner < r_T < radius.
+__device__ inline bool plane_pass(const DetectorPlanes& D, int d, double x0, double y0, double z0, double px, double py,
+                                  double pz) {
+#pragma clang fp contract(off)
+  const double T = (D.z[d] - z0) / pz;
+  const double xf = x0 + T * px, yf = y0 + T * py;
+  const double rT = sqrt(xf * xf + yf * yf);
+  return rT > D.inner && rT < D.radius;
+}
+
+// Grid-stride over the records.  With at most kRegPlanes planes every thread
+// keeps its weight sums in registers and they are reduced once per wave at
+// the end; otherwise each block pass reduces them per wave.  Per block the
+// wave sums meet in LDS, then one float64 atomic per block and plane.
+constexpr int kRegPlanes = 8;
+template <bool REG>
+__global__ __launch_bounds__(256) void k_detector_cut(const float4* __restrict__ p, const float4* __restrict__ r,
+                                                      int64_t n, DetectorPlanes D, uint64_t* __restrict__ mask,
+                                                      double* __restrict__ wsum) {
+  __shared__ double s_w[PETITE_MAX_DETECTORS + 1];
+  for (int d = threadIdx.x; d <= D.n; d += blockDim.x) s_w[d] = 0.0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  double acc[kRegPlanes + 1];
+#pragma unroll
+  for (int d = 0; d <= kRegPlanes; ++d) acc[d] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+    const int64_t i = i0 + threadIdx.x;
+    bool inwin = false;
+    double w = 0.0, x0 = 0.0, y0 = 0.0, z0 = 0.0, px = 0.0, py = 0.0, pz = 1.0;
+    if (i < n) {
+      const float4 pi = p[i], ri = r[i];
+      const double E = (double)pi.x;
+      inwin = !D.has_ecut || (E < D.emax && E > D.emin);
+      w = (double)ri.w;
+      x0 = (double)ri.x;
+      y0 = (double)ri.y;
+      z0 = (double)ri.z;
+      px = (double)pi.y;
+      py = (double)pi.z;
+      pz = (double)pi.w;
+    }
+    uint64_t bits = 0;
+    if (REG) {
+#pragma unroll
+      for (int d = 0; d < kRegPlanes; ++d) {
+        if (d < D.n) {
+          const bool pass = inwin && plane_pass(D, d, x0, y0, z0, px, py, pz);
+          bits |= (uint64_t)pass << d;
+          acc[d] += pass ? w : 0.0;
+        }
+      }
+      acc[kRegPlanes] += inwin ? w : 0.0;
+    } else {
+      for (int d = 0; d < D.n; ++d) {
+        const bool pass = inwin && plane_pass(D, d, x0, y0, z0, px, py, pz);
+        bits |= (uint64_t)pass << d;
+        const double v = wave_sum_f64(pass ? w : 0.0);
+        if (lane == 0 && v != 0.0) atomicAdd(&s_w[d], v);
+      }
+      const double t = wave_sum_f64(inwin ? w : 0.0);
+      if (lane == 0 && t != 0.0) atomicAdd(&s_w[D.n], t);
+    }
+    if (mask && i < n) mask[i] = bits;
+  }
+  if (REG) {
+#pragma unroll
+    for (int d = 0; d <= kRegPlanes; ++d) {
+      const int slot = d == kRegPlanes ? D.n : d;
+      if (d < D.n || d == kRegPlanes) {
+        const double v = wave_sum_f64(acc[d]);
+        if (lane == 0 && v != 0.0) atomicAdd(&s_w[slot], v);
+      }
+    }
+  }
+  __syncthreads();
+  if (wsum)
+    for (int d = threadIdx.x; d <= D.n; d += blockDim.x)
+      if (s_w[d] != 0.0) unsafeAtomicAdd(wsum + d, s_w[d]);
+}
 
 }  // namespace
 
@@ -1762,6 +1860,34 @@ int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t
       if (dyield_host) dyield_host[order[k0 + k]] = h[2 * k + 1];
     }
     k0 += P.n;
+  }
+  return PETITE_OK;
+}
+
+int petite_detector_cut(const float* p, const float* r, int64_t n, const double* det_z, int32_t n_det, double radius,
+                        double inner_radius, int32_t has_ecut, double emin, double emax, uint64_t* mask, double* wsum,
+                        PetiteStream stream) {
+  if (n < 0 || n_det < 1 || n_det > PETITE_MAX_DETECTORS || !det_z || (n > 0 && (!p || !r)))
+    return fail(PETITE_ERR_ARG, "petite_detector_cut: bad arguments");
+  hipStream_t s = (hipStream_t)stream;
+  DetectorPlanes D{};
+  D.n = n_det;
+  D.has_ecut = has_ecut ? 1 : 0;
+  D.radius = radius;
+  D.inner = inner_radius;
+  D.emin = emin;
+  D.emax = emax;
+  for (int d = 0; d < n_det; ++d) D.z[d] = det_z[d];
+  if (wsum) HIP_TRY(hipMemsetAsync(wsum, 0, (size_t)(n_det + 1) * sizeof(double), s));
+  if (n > 0) {
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+    if (n_det <= kRegPlanes)
+      hipLaunchKernelGGL(k_detector_cut<true>, dim3((unsigned)blocks), dim3(256), 0, s, (const float4*)p,
+                         (const float4*)r, n, D, mask, wsum);
+    else
+      hipLaunchKernelGGL(k_detector_cut<false>, dim3((unsigned)blocks), dim3(256), 0, s, (const float4*)p,
+                         (const float4*)r, n, D, mask, wsum);
+    HIP_TRY(hipGetLastError());
   }
   return PETITE_OK;
 }

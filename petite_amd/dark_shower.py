@@ -248,8 +248,11 @@ class DarkShower(Shower):
             print("Need an existing SM shower-file directory or SM incident particle to run dark shower")
             return None
         if ExDir is not None and isinstance(ExDir, str):
-            raise NotImplementedError("ExDir as a file path: the reference's pickled Particle lists are not "
-                                      "loaded; pass the list of Particle or a ParticleBatch")
+            if not ExDir.endswith(".npz"):
+                raise NotImplementedError("ExDir as a file path: the reference's pickled Particle lists are not "
+                                          "loaded; pass a record written by ParticleBatch.save (.npz), the list "
+                                          "of Particle or a ParticleBatch")
+            ExDir = ParticleBatch.load(ExDir)
         if ExDir is not None and isinstance(ExDir, ParticleBatch):
             dark = self.generate_dark_showers(ExDir)
             sm = ExDir.to_particles()

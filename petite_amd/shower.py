@@ -21,6 +21,7 @@ from . import _lib
 from .batch import PRIMARY_DTYPE, ParticleBatch
 from .constants import (all_process_code, cmtom, dimensionalities, mass_dict, process_code, process_PIDS,
                         target_information)
+from .detector import detector_cut, transverse_position  # noqa: F401  (module functions of shower.py:815-865)
 from .device import DeviceContext
 from .particle import Particle
 from .tables import MaterialTables, table_set_for

@@ -59,3 +59,18 @@ def test_error_string_without_gpu():
 def test_primary_dtype_matches_struct():
     from petite_amd.batch import PRIMARY_DTYPE
     assert PRIMARY_DTYPE.itemsize == ctypes.sizeof(_lib.Primary)
+
+
+def test_detector_cut_rejects_bad_arguments_without_gpu():
+    lib = _lib.load()
+    z = (ctypes.c_double * 1)(1.0)
+    assert lib.petite_detector_cut(None, None, 0, z, 0, 1.0, 0.0, 0, 0.0, 0.0, None, None, None) == _lib.PETITE_ERR_ARG
+    assert lib.petite_detector_cut(None, None, 5, z, 1, 1.0, 0.0, 0, 0.0, 0.0, None, None, None) == _lib.PETITE_ERR_ARG
+    assert lib.petite_detector_cut(None, None, 0, None, 1, 1.0, 0.0, 0, 0.0, 0.0, None, None, None) == _lib.PETITE_ERR_ARG
+    assert b"petite_detector_cut" in lib.petite_last_error()
+
+
+def test_detector_limit_matches_header():
+    from petite_amd import detector
+    m = re.search(r"#define PETITE_MAX_DETECTORS (\d+)", open(HEADER).read())
+    assert int(m.group(1)) == detector.MAX_DETECTORS
