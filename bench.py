@@ -193,6 +193,8 @@ def main():
     det_out = bench_detector(b, cpu=not args.no_cpu_baseline)
     print(f"detector: {det_out['records']} records, {det_out['ms']:.3f} ms", file=sys.stderr, flush=True)
     del b
+    train_out = bench_map_training(cpu=not args.no_cpu_baseline)
+    print(f"map training: {train_out['samples']} samples, {train_out['s_total']:.2f} s", file=sys.stderr, flush=True)
 
     dark_out = yield_out = None
     if args.dark_forward:
@@ -241,6 +243,7 @@ def main():
     if cpu:
         out["gpu_over_cpu_showers"] = showers_per_s / cpu["showers_per_s"]
     out["detector_cut"] = det_out
+    out["map_training"] = train_out
     if dark_out:
         out["dark_forward"] = dark_out
     if yield_out:
@@ -286,6 +289,50 @@ def bench_detector(batch, reps=10, cpu=True):
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": m / dt, "unit": "records/s", "cores": 1, "kind": "port",
                                "sample": f"first {m} records, {dt:.2f} s (oracle numpy float64)"}
+    return out
+
+
+def bench_map_training(n_iter=10, n_per=1 << 16, cpu=True):
+    """VEGAS map training (vegas_train): the 400 GeV Brem map, missing from the
+    shipped tables, put on the 150 cross-section energies of the 400 GeV set
+    (lead) from the nearest 100 GeV maps and trained; the accept/reject
+    efficiency (mean J f / max J f) above 100 GeV before and after."""
+    import torch
+
+    from petite_amd import Shower
+    sh = Shower("400GeV", "lead", 0.100, seed=SEED)
+    E0 = sh.device_context().tables_grid("Brem")[0]
+    # the stand-in's nodes on the 400 GeV energies (nearest copy, no adaptation)
+    sh.train_maps(["Brem"], energies="sigma", n_iter=0)
+    ctx = sh.device_context()
+    E = ctx.tables_grid("Brem")[0]
+    hi = E > 100.0
+    eff_copy = ctx.sigma_est["Brem"] / ctx.maxf["Brem"]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sh.train_maps(["Brem"], n_iter=n_iter, n_per_energy=n_per)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    eff = ctx.sigma_est["Brem"] / ctx.maxf["Brem"]
+    samples = len(E) * n_iter * n_per
+    out = {"workload": f"400 GeV Brem map (lead): {len(E)} energies to {E[-1]:.0f} GeV (shipped stand-in: "
+                       f"{len(E0)} to {E0[-1]:.0f} GeV), {n_iter} adaptations of {n_per} samples per energy",
+           "samples": samples, "s_total": dt, "samples_per_s": samples / dt,
+           "efficiency_above_100GeV_stand_in": float(np.median(eff_copy[hi])),
+           "efficiency_above_100GeV_trained": float(np.median(eff[hi]))}
+    if cpu:
+        from oracle import vegas as ov
+        from oracle.tables import SMTables
+        T = SMTables("lead")
+        grid = T.grids["Brem"]
+        ie = len(grid.E) - 1
+        m = 1 << 19
+        t0 = time.perf_counter()
+        ov.train_accumulate(grid, "Brem", ie, T.event_info("Brem", grid.E[ie]), 7, 0, m)
+        t1 = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": m / t1, "unit": "samples/s", "cores": 1, "kind": "port",
+                               "sample": f"{m} samples of the accumulation at one energy, {t1:.3f} s (oracle numpy "
+                                         "float64; the adaptation itself is host code in both)"}
     return out
 
 

@@ -254,6 +254,21 @@ int petite_yield_setup(PetiteCtx* ctx, int32_t n_mass, const double* masses, con
 int petite_dark_yield(PetiteCtx* ctx, const PetiteStore* sm, int64_t n_sm, int32_t n_points, const double* mV_host,
                       double* yield_host, double* dyield_dmV_host, PetiteStream stream);
 
+/* VEGAS map training (the reference trains its maps with the vegas package:
+ * utilities/generate_integrators.py:49-100, options at all_processes.py:
+ * 1085-1096).  petite_map_accumulate adds, for every grid energy of process
+ * proc, n_per_energy samples uniform in y mapped through the context's
+ * current grid: sum_f[ie*stride + node_offset(d) + i] += (J f)^2 and
+ * n_f[...] += 1 for the increment i of each dimension d the sample falls in
+ * (device arrays of n_E * stride, zeroed by the caller; non-finite J f are
+ * skipped).  Sample k of iteration it uses Philox block it*n_per_energy + k
+ * of stream (proc << 32 | ie), purpose PURPOSE_MAP_TRAIN = 9.
+ * petite_set_grid replaces the context's nodes of proc (host array of
+ * n_E * stride floats, the layout of PetiteGridDesc); rescan max_F after. */
+int petite_map_accumulate(PetiteCtx* ctx, int32_t proc, int64_t n_per_energy, uint64_t seed, uint32_t iteration,
+                          double* sum_f, uint32_t* n_f, PetiteStream stream);
+int petite_set_grid(PetiteCtx* ctx, int32_t proc, const float* nodes_host, int64_t n);
+
 /* Detector acceptance of straight-line extrapolations: replaces
  * transverse_position and detector_cut (src/PETITE/shower.py:815-865).
  * p, r: n records of float4 (E, px, py, pz) and (x, y, z, weight) on the

@@ -28,6 +28,7 @@ PURPOSE_VEGAS_A = 4    # candidate j: y0..y3 (ndim<=3: u3 is the accept uniform;
 PURPOSE_CHILD = 6      # index k: child stream id (words 0,1)
 PURPOSE_DARK = 7       # index = dark process code: dark stream id (words 0,1)
 PURPOSE_BOUND_E = 8    # index j: bound-electron momentum accept/reject pairs
+PURPOSE_MAP_TRAIN = 9  # map training sample j of (process, energy): y0..y3
 
 
 def philox4x32_10(ctr, key):

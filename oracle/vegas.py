@@ -91,3 +91,73 @@ def draw(grid, process, E, ev, max_f, seed, sid, max_trials=3_000_000, chunk=256
             return x[k], j0 + k + 1, float(w[k])
         j0 += m
     return None, max_trials, 0.0
+
+
+# ------------------------------------------------------------ map training --
+def train_accumulate(grid, process, ie, ev, seed, iteration, n):
+    """(sum_f, n_f) of one grid energy, in the node layout of the grid (one
+    unused slot per dimension): samples k = 0..n-1 of iteration `iteration`
+    take Philox block iteration*n + k of stream (code << 32 | ie), purpose
+    MAP_TRAIN; per dimension the sample's increment gets (J f)^2 and a count
+    (the data vegas's AdaptiveMap.adapt consumes; utilities/
+    generate_integrators.py:66-74 trains the reference's maps with it)."""
+    code = PROCESS_CODE[process]
+    sid = np.uint64((code << 32) | ie)
+    j = np.arange(n, dtype=np.uint64) + np.uint64(iteration) * np.uint64(n)
+    w = philox.words(seed, sid, j, philox.PURPOSE_MAP_TRAIN)
+    ndim = grid.ndim
+    y = philox.to_uniform(w)[:, :ndim]
+    x, J = map_points(grid, ie, y)
+    f = J * INTEGRAND[process](x, ev)
+    f2 = f * f
+    ok = np.isfinite(f2)
+    sizes = [len(grid.dim_nodes(ie, d)) for d in range(ndim)]
+    sum_f = np.zeros(sum(sizes))
+    n_f = np.zeros(sum(sizes), dtype=np.int64)
+    off = 0
+    for d in range(ndim):
+        N = sizes[d] - 1
+        i = np.floor(y[:, d] * N).astype(np.int64)
+        np.add.at(sum_f, off + i[ok], f2[ok])
+        np.add.at(n_f, off + i[ok], 1)
+        off += N + 1
+    return sum_f, n_f
+
+
+def refine(nodes, sum_f, n_f, alpha=0.5):
+    """One vegas map adaptation of one dimension (AdaptiveMap.adapt; Lepage,
+    J. Comput. Phys. 27 (1978) 192 and 439 (2021) 110386), written as its
+    sequential loops: per-increment average of (J f)^2, smoothing
+    (7,1)/8 at the ends and (1,6,1)/8 inside, normalisation, damping
+    ((1 - t)/ln(1/t))^alpha, then the walk that places node k where the
+    cumulative damped importance reaches k/N of its total."""
+    nodes = [float(v) for v in nodes]
+    N = len(nodes) - 1
+    avg = [sum_f[i] / n_f[i] if n_f[i] > 0 else 0.0 for i in range(N)]
+    if alpha > 0 and N > 1:
+        t = [0.0] * N
+        t[0] = (7.0 * avg[0] + avg[1]) / 8.0
+        t[N - 1] = (7.0 * avg[N - 1] + avg[N - 2]) / 8.0
+        for i in range(1, N - 1):
+            t[i] = (6.0 * avg[i] + avg[i - 1] + avg[i + 1]) / 8.0
+    else:
+        t = list(avg)
+    total = sum(t)
+    if not (total > 0.0) or not np.isfinite(total):
+        return np.array(nodes)
+    t = [v / total for v in t]
+    if alpha > 0:
+        t = [((1.0 - v) / np.log(1.0 / v)) ** alpha if 0.0 < v < 1.0 else (1.0 if v >= 1.0 else 0.0) for v in t]
+    per = sum(t) / N
+    new = [nodes[0]]
+    acc = 0.0   # damped importance accumulated up to the start of increment j
+    j = 0
+    for k in range(1, N):
+        goal = k * per
+        while j < N - 1 and acc + t[j] <= goal:
+            acc += t[j]
+            j += 1
+        frac = (goal - acc) / t[j] if t[j] > 0 else 0.0
+        new.append(nodes[j] + frac * (nodes[j + 1] - nodes[j]))
+    new.append(nodes[N])
+    return np.array(new)

@@ -19,7 +19,8 @@ PETITE_ERR_CAPACITY = 4
 
 EXPORTED = ["petite_ctx_create", "petite_ctx_destroy", "petite_last_error", "petite_maxf_scan", "petite_set_maxf",
             "petite_vegas_draw", "petite_eval_points", "petite_run_showers", "petite_set_timing", "petite_dark_setup",
-            "petite_dark_reweight", "petite_yield_setup", "petite_dark_yield", "petite_detector_cut"]
+            "petite_dark_reweight", "petite_yield_setup", "petite_dark_yield", "petite_detector_cut",
+            "petite_map_accumulate", "petite_set_grid"]
 MAX_MASSES = 8
 DARK_FAMILIES = ["brem_elec", "brem_positron", "muon_brem", "annihilation"]
 DARK_DRATE_BINS = 10
@@ -140,6 +141,10 @@ def load():
     f64 = C.c_double
     lib.petite_detector_cut.argtypes = [vp, vp, i64, vp, i32, f64, f64, i32, f64, f64, vp, vp, vp]
     lib.petite_detector_cut.restype = C.c_int
+    lib.petite_map_accumulate.argtypes = [vp, i32, i64, u64, C.c_uint32, vp, vp, vp]
+    lib.petite_map_accumulate.restype = C.c_int
+    lib.petite_set_grid.argtypes = [vp, i32, vp, i64]
+    lib.petite_set_grid.restype = C.c_int
     _lib = lib
     return lib
 

@@ -340,6 +340,41 @@ __global__ void k_bucket(DevTables T, const int32_t* proc, const double* E, int6
   req_E[(int64_t)p * req_cap + k] = E[i];
 }
 
+// ----------------------------------------------------------- map training --
+// VEGAS adaptation data (the reference trains its maps with the vegas package,
+// utilities/generate_integrators.py:49-100, vegas_integrator_options at
+// all_processes.py:1085-1096; published algorithm: Lepage 1978, 2021).  At each
+// grid energy, samples uniform in y pass through the current map; per
+// increment of every dimension the kernel sums (J f)^2 and counts the samples.
+// The increment index is the map's integer-exact one.  Block (ie, chunk).
+__global__ __launch_bounds__(256) void k_map_accumulate(DevTables T, int proc, int64_t n_per, uint64_t seed,
+                                                        uint32_t iteration, double* sum_f, uint32_t* n_f) {
+  const int ie = blockIdx.x;
+  const PetiteGridDesc& g = T.d->grid[proc];
+  const double E = T.f64[g.E_off + ie];
+  const uint64_t sid = ((uint64_t)proc << 32) | (uint64_t)ie;
+  const int64_t step = (int64_t)gridDim.y * blockDim.x;
+  for (int64_t k = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; k < n_per; k += step) {
+    const uint32_t j = (uint32_t)((uint64_t)iteration * (uint64_t)n_per + (uint64_t)k);
+    const U4 a = draw_block(seed, sid, j, PURPOSE_MAP_TRAIN);
+    const uint32_t rr[4] = {a.x, a.y, a.z, a.w};
+    double x[4] = {0.0, 0.0, 0.0, 0.0};
+    const double J = map_point(T, proc, ie, rr, g.ndim, x);
+    const double f = J * integrand(T, proc, E, x);
+    const double f2 = f * f;
+    if (!isfinite(f2)) continue;
+    int64_t off = (int64_t)ie * g.stride;
+    for (int d = 0; d < g.ndim; ++d) {
+      const uint32_t N = (uint32_t)g.ninc[d];
+      const uint64_t q = (uint64_t)(2u * (rr[d] >> 9) + 1u) * (uint64_t)N;
+      const int64_t i = off + (int64_t)(q >> 24);
+      unsafeAtomicAdd(sum_f + i, f2);
+      atomicAdd(n_f + i, 1u);
+      off += N + 1;
+    }
+  }
+}
+
 // ------------------------------------------------------------- max_F scan --
 // One block per grid energy; every thread evaluates a strided share of the
 // n_samples candidates at the grid energy (find_maxes.py:102-110).
@@ -1419,6 +1454,32 @@ int petite_maxf_scan(PetiteCtx* c, int32_t proc, int32_t n_samples, uint64_t see
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   (void)hipFree(dmean);
   if (e != hipSuccess) return fail(PETITE_ERR_HIP, std::string("petite_maxf_scan: ") + hipGetErrorString(e));
+  return PETITE_OK;
+}
+
+int petite_map_accumulate(PetiteCtx* c, int32_t proc, int64_t n_per_energy, uint64_t seed, uint32_t iteration,
+                          double* sum_f, uint32_t* n_f, PetiteStream stream) {
+  if (!c || proc < 0 || proc >= PETITE_NPROC || n_per_energy <= 0 || !sum_f || !n_f)
+    return fail(PETITE_ERR_ARG, "petite_map_accumulate: bad arguments");
+  const PetiteGridDesc& g = c->desc.grid[proc];
+  if (g.n_E <= 0) return fail(PETITE_ERR_ARG, "petite_map_accumulate: process has no grid");
+  if ((uint64_t)(iteration + 1ull) * (uint64_t)n_per_energy > 0xFFFFFFFFull)
+    return fail(PETITE_ERR_ARG, "petite_map_accumulate: iteration * n_per_energy exceeds the 32-bit sample index");
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t chunks = std::max<int64_t>(1, std::min<int64_t>((n_per_energy + 1023) / 1024, 64));
+  hipLaunchKernelGGL(k_map_accumulate, dim3((unsigned)g.n_E, (unsigned)chunks), dim3(256), 0, s, c->dev, proc,
+                     n_per_energy, seed, iteration, sum_f, n_f);
+  HIP_TRY(hipGetLastError());
+  return PETITE_OK;
+}
+
+int petite_set_grid(PetiteCtx* c, int32_t proc, const float* nodes_host, int64_t n) {
+  if (!c || proc < 0 || proc >= PETITE_NPROC || !nodes_host) return fail(PETITE_ERR_ARG, "petite_set_grid: bad arguments");
+  const PetiteGridDesc& g = c->desc.grid[proc];
+  if (g.n_E <= 0 || n != (int64_t)g.n_E * g.stride)
+    return fail(PETITE_ERR_ARG, "petite_set_grid: node count differs from the process's grid");
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(c->d32 + g.node_off, nodes_host, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
   return PETITE_OK;
 }
 

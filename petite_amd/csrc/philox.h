@@ -18,6 +18,7 @@ enum Purpose : uint32_t {
   PURPOSE_CHILD = 6,       // index k: child stream id
   PURPOSE_DARK = 7,        // index = dark process code: dark stream id
   PURPOSE_BOUND_E = 8,     // bound-electron momentum accept/reject pairs
+  PURPOSE_MAP_TRAIN = 9,   // map training sample j of (process, energy): y0..y3
 };
 
 struct U4 {

@@ -78,6 +78,29 @@ class DeviceContext:
         self.sigma_est[process] = mean
         return mx, mean
 
+    def tables_grid(self, process):
+        """(E, nodes float32, node counts) of the process's map in this context."""
+        if process in self.tables.grids:
+            return self.tables.grids[process]
+        if self.dark is not None and process in self.dark.grids:
+            return self.dark.grids[process]
+        raise KeyError(f"{process}: no map in this context")
+
+    def set_grid(self, process, nodes):
+        """Replace the process's map nodes (n_E * stride float32) and rescan max_F."""
+        code = all_process_code[process]
+        v = np.ascontiguousarray(nodes, dtype=np.float32).ravel()
+        _lib.check(self.lib.petite_set_grid(self.ctx, code, v.ctypes.data, v.size), "petite_set_grid")
+        self.scan_maxf(process)
+
+    def train_map(self, process, **kw):
+        """Train the process's map in this context (vegas_train.train_process)
+        and rescan max_F; returns the new nodes (n_E, stride) float32."""
+        from .vegas_train import train_process
+        nodes = train_process(self, process, **kw)
+        self.scan_maxf(process)
+        return nodes
+
     def set_maxf(self, process, values):
         code = all_process_code[process]
         v = np.ascontiguousarray(values, dtype=np.float64)

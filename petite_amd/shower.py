@@ -197,6 +197,38 @@ class Shower:
         return b0 / (b0 + b1)
 
     # -------------------------------------------------------------- device --
+    def train_maps(self, processes=None, n_iter=10, n_per_energy=1 << 16, alpha=0.5, seed=None, energies=None):
+        """Retrain the VEGAS maps of the given processes (default: every process
+        with a map) on the GPU, starting from the current ones, and rescan
+        max_F (petite_amd.vegas_train).  The reference ships its maps trained
+        with the vegas package (utilities/generate_integrators.py); a retrained
+        map keeps the end points and changes only the draws' efficiency.
+        energies="sigma" first puts each SM map on its process's cross-section
+        energies (e.g. up to 400 GeV in the 400 GeV set, whose Brem/PairProd/
+        MuonBrem maps are missing); an array puts every map on those energies."""
+        from .vegas_train import TRAIN_SEED
+        if energies is not None:
+            if processes is None or (isinstance(energies, str) and energies != "sigma"):
+                raise ValueError('train_maps: energies needs the processes, and takes "sigma" or an array')
+            for P in processes:
+                E_new = self._tables.nsigma_tab[P][0] if isinstance(energies, str) else energies
+                self._tables.regrid_map(P, E_new)
+            for c in self._ctx.values():
+                c.close()
+            self._ctx = {}
+        ctx = self.device_context()
+        if processes is None:
+            processes = [P for P in all_process_code if ctx.desc.grid[all_process_code[P]].n_E > 0]
+        for P in processes:
+            E, _, cnt = ctx.tables_grid(P)
+            nodes = ctx.train_map(P, n_iter=n_iter, n_per_energy=n_per_energy, alpha=alpha,
+                                  seed=TRAIN_SEED if seed is None else seed)
+            grids = self._tables.grids if P in self._tables.grids else self._dark.grids
+            grids[P] = (E, nodes.ravel(), cnt)
+            for other in self._ctx.values():
+                if other is not ctx:
+                    other.set_grid(P, nodes)
+
     def device_context(self, MS=True):
         key = bool(MS)
         if key not in self._ctx:
