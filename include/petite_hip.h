@@ -177,7 +177,8 @@ typedef struct {
   int32_t d_n[PETITE_DARK_FAMILIES];   /* drate tables (_d_rate_d_E_*, 311-593): saved energies E[n] at d_off, */
   int64_t d_off[PETITE_DARK_FAMILIES]; /* then bin counts cnt[n], then rows [n][10][2] (bin energy, rate) */
   int32_t comp_n;                      /* DarkComp n*sigma [1/cm]: log10 E[n] at comp_off, log10 value[n] after */
-  int32_t n_active;                    /* active processes, in the caller's order (dark process codes 8..11) */
+  int32_t n_active;                    /* active processes, in the caller's order (dark process codes 8..11, and
+                                          * 12 = TwoBody_BSMDecay of pi0 / eta / eta' records) */
   int64_t comp_off;
   int32_t active[8];
   double eps2;                         /* g_e^2 / (4 pi alpha) = kinetic_mixing^2 */
@@ -188,6 +189,10 @@ typedef struct {
   double w_lnE0[PETITE_DARK_FAMILIES];
   double w_inv_dlog[PETITE_DARK_FAMILIES];
   double comp_l0, comp_inv_dl;
+  /* TwoBody_BSMDecay (dark_shower.py:633-640): 2 kappa^2 BR(M -> gamma gamma)
+   * of pi0, eta, eta' (0 unless code 12 is active); the weight of a meson of
+   * mass m_M is meson_w (1 - (m_V/m_M)^2)^3 for m_V < m_M */
+  double meson_w[3];
 } PetiteDarkDesc;
 
 /* Dark vectors written by petite_dark_reweight, one entry per (SM particle,
@@ -212,7 +217,8 @@ typedef struct {
   int64_t n_zero_rate;  /* drate rows that sum to zero (722-750) */
   int64_t n_pe_fail;    /* bound-electron momentum draws that found no sample in 65536 pairs */
   int64_t pad;
-  double yield[4];      /* sum of vector weights per dark process (DarkBrem, DarkAnn, DarkComp, DarkMuonBrem) */
+  double yield[5];      /* sum of vector weights per dark process (DarkBrem, DarkAnn, DarkComp, DarkMuonBrem,
+                         * TwoBody_BSMDecay) */
   double ms_select, ms_draw, ms_emit;
   int64_t launches;
 } PetiteDarkStats;

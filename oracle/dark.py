@@ -496,3 +496,60 @@ def yield_and_grad(samplers, masses, processes, pid, E0, w, mV):
         Y.append(float(np.sum(W.sum(0) * w)))
         dY.append(float(np.sum(dW.sum(0) * w)))
     return np.array(Y), np.array(dY)
+
+
+# ------------------------------------------------------------- meson decays --
+MESON_MASS = {111: 134.9768e-3, 221: 547.862e-3, 331: 957.78e-3}
+MESON_BR_GG = {111: 0.98823, 221: 0.3936, 331: 0.02307}   # particle.py:30-48
+TWO_BODY_CODE = 12
+
+
+def meson_weight(pid, m_meson, mV, kinetic_mixing=1.0):
+    """GetBSMWeights of a meson for TwoBody_BSMDecay (dark_shower.py:633-640)."""
+    r = mV / m_meson
+    if r >= 1.0:
+        return 0.0
+    return 2 * kinetic_mixing ** 2 * (1.0 - r ** 2) ** 3 * MESON_BR_GG[pid]
+
+
+def boost_matrix(pf, m0):
+    """Particle.boost_matrix (particle.py:187-207): rest frame -> lab along pf."""
+    E0, px0, py0, pz0 = (float(v) for v in pf)
+    gamma = E0 / m0
+    beta = 1.0 if gamma == 1.0 else np.sqrt(1.0 - 1.0 / gamma ** 2)
+    pmag = np.linalg.norm([px0, py0, pz0])
+    if pmag == 0.0:
+        return np.identity(4)
+    bx, by, bz = beta * np.array([px0, py0, pz0]) / pmag
+    b2 = beta ** 2
+    return np.array([[gamma, gamma * bx, gamma * by, gamma * bz],
+                     [gamma * bx, 1 + (gamma - 1) * bx ** 2 / b2, (gamma - 1) * bx * by / b2, (gamma - 1) * bx * bz / b2],
+                     [gamma * by, (gamma - 1) * by * bx / b2, 1 + (gamma - 1) * by ** 2 / b2, (gamma - 1) * by * bz / b2],
+                     [gamma * bz, (gamma - 1) * bz * bx / b2, (gamma - 1) * bz * by / b2, 1 + (gamma - 1) * bz ** 2 / b2]])
+
+
+def two_body_vector(pf, m_meson, mV, cos_theta, phi):
+    """Lab 4-vector of V in M -> gamma V (two_body_decay, particle.py:209-254,
+    gamma first, V second), for given rest-frame angles."""
+    E2 = (m_meson ** 2 - 0.0 + mV ** 2) / (2 * m_meson)
+    E1 = (m_meson ** 2 - mV ** 2 + 0.0) / (2 * m_meson)
+    pF = np.sqrt(E1 ** 2)
+    sin_theta = np.sqrt(1 - cos_theta ** 2)
+    p2_rf = [E2, pF * sin_theta * np.sin(phi), pF * sin_theta * np.cos(phi), pF * cos_theta]
+    return np.dot(boost_matrix(pf, m_meson), p2_rf)
+
+
+def produce_meson(rec, mV, seed, kinetic_mixing=1.0):
+    """The dark vector of one meson record (dark_shower.py:836-844) with the
+    angles from its dark stream dsid = derive_sid(seed, sid, 12, DARK):
+    INTERACT block 0, cos theta = 2x - 1, phi = 2 pi y.  Returns None when
+    m_V >= m_M, else {"p", "r", "weight"}.  The meson's mass is its PDG value."""
+    pid = rec["pid"]
+    m = MESON_MASS[pid]
+    wg = meson_weight(pid, m, mV, kinetic_mixing)
+    if wg <= 0.0:
+        return None
+    dsid = philox.derive_sid(seed, rec["sid"], TWO_BODY_CODE, philox.PURPOSE_DARK)
+    u = philox.block(seed, dsid, 0, philox.PURPOSE_INTERACT)
+    p = two_body_vector(rec["pf"], m, mV, -1.0 + 2.0 * u[0], 2.0 * np.pi * u[1])
+    return {"p": p, "r": np.asarray(rec["rf"], dtype=np.float64), "weight": wg * rec["weight"]}

@@ -73,7 +73,7 @@ class DarkDesc(C.Structure):
                 ("eps2", C.c_double), ("E_res", C.c_double), ("E_C", C.c_double), ("min_brem", C.c_double),
                 ("min_ann", C.c_double), ("min_comp", C.c_double), ("min_muon_brem", C.c_double),
                 ("w_lnE0", C.c_double * 4), ("w_inv_dlog", C.c_double * 4), ("comp_l0", C.c_double),
-                ("comp_inv_dl", C.c_double)]
+                ("comp_inv_dl", C.c_double), ("meson_w", C.c_double * 3)]
 
 
 class DarkOut(C.Structure):
@@ -84,7 +84,7 @@ class DarkOut(C.Structure):
 class DarkStats(C.Structure):
     _fields_ = [("n_entries", C.c_int64), ("n_vectors", C.c_int64), ("n_draws", C.c_int64), ("n_trials", C.c_int64),
                 ("n_no_sample", C.c_int64), ("n_zero_rate", C.c_int64), ("n_pe_fail", C.c_int64), ("pad", C.c_int64),
-                ("yield_", C.c_double * 4), ("ms_select", C.c_double), ("ms_draw", C.c_double),
+                ("yield_", C.c_double * 5), ("ms_select", C.c_double), ("ms_draw", C.c_double),
                 ("ms_emit", C.c_double), ("launches", C.c_int64)]
 
     def as_dict(self):

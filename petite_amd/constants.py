@@ -57,3 +57,7 @@ process_PIDS = {"PairProd": [-11, 11], "Brem": [0, 22], "MuonBrem": [0, 22], "Co
                 "Moller": [0, 11], "Bhabha": [0, 11], "MuonE": [0, 11]}
 dark_process_codes = ["DarkBrem", "DarkAnn", "DarkComp", "TwoBody_BSMDecay", "DarkMuonBrem"]
 PID_DARK_VECTOR = 4900022
+# TwoBody_BSMDecay: dark process code of its output entries, and BR(M -> gamma gamma)
+# of the mesons it applies to (particle.py:30-48)
+TWO_BODY_CODE = 12
+meson_twobody_branchingratios = {111: 0.98823, 221: 0.3936, 331: 0.02307}

@@ -451,6 +451,84 @@ __global__ __launch_bounds__(256) void k_dark_emit(DevTables T, PetiteDarkDesc D
   wave_stat(stats, 2, n_pe_fail);
 }
 
+// ------------------------------------------------------ meson decays ------
+constexpr int TWO_BODY = 12;  // TwoBody_BSMDecay: dark process code of the output entries
+
+// TwoBody_BSMDecay of pi0 / eta / eta' records (dark_shower.py:633-640,
+// 836-844): weight meson_w (1 - r^2)^3 with r = m_V / m_M, and the vector of
+// the decay M -> gamma V, isotropic in the meson's rest frame (cos theta and
+// phi uniform; particle.py:209-254 with V second) and boosted along the
+// meson's final momentum with the boost_matrix of particle.py:187-207; the
+// vector sits at the meson's end point.  Meson masses are the PDG values of
+// mass_dict.  Entries are appended after out_base through *count; an entry
+// beyond capacity is counted but not written.
+__global__ __launch_bounds__(256) void k_meson_decay(DevTables T, PetiteDarkDesc D, DevStore S, int64_t first,
+                                                     int64_t n, uint64_t seed, EmitOut O, int64_t out_base,
+                                                     int64_t capacity, unsigned long long* count, double* yield,
+                                                     unsigned long long* stats) {
+  const int64_t i = first + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  bool have = false;
+  double w = 0.0;
+  float4 pv = make_float4(0.f, 0.f, 0.f, 0.f), rv = pv;
+  if (i < first + n) {
+    const int pid = S.pid[i];
+    const int k = pid == 111 ? 0 : (pid == 221 ? 1 : (pid == 331 ? 2 : -1));
+    const double mM = k == 0 ? 0.1349768 : (k == 1 ? 0.547862 : 0.95778);
+    const double mV = T.d->mV;
+    if (k >= 0 && D.meson_w[k] > 0.0 && mV / mM < 1.0) {
+      const double r = mV / mM;
+      const double q = 1.0 - r * r;
+      const double wg = D.meson_w[k] * q * q * q;
+      w = wg * (double)S.r0[i].w;
+      const uint64_t dsid = derive_sid(seed, S.sid[i], (uint32_t)TWO_BODY, PURPOSE_DARK);
+      const U4 u = draw_block(seed, dsid, 0, PURPOSE_INTERACT);
+      const double ct = -1.0 + 2.0 * u01d(u.x);
+      const double phi = 2.0 * kPi * u01d(u.y);
+      const double st = sqrt(1.0 - ct * ct);
+      const double E2 = (mM * mM + mV * mV) / (2.0 * mM);
+      const double pF = (mM * mM - mV * mV) / (2.0 * mM);  // the photon's energy = |p| (m_gamma = 0)
+      double sp, cp;
+      sincos(phi, &sp, &cp);
+      const double v[4] = {E2, pF * st * sp, pF * st * cp, pF * ct};
+      const float4 pf = S.pf[i];
+      const double E0 = pf.x, px = pf.y, py = pf.z, pz = pf.w;
+      const double gam = E0 / mM;
+      const double beta = gam == 1.0 ? 1.0 : sqrt(1.0 - 1.0 / (gam * gam));
+      const double pm = sqrt(px * px + py * py + pz * pz);
+      double o[4] = {v[0], v[1], v[2], v[3]};
+      if (pm != 0.0) {
+        const double b[3] = {beta * px / pm, beta * py / pm, beta * pz / pm};
+        const double g1 = (gam - 1.0) / (beta * beta);
+        const double bv = b[0] * v[1] + b[1] * v[2] + b[2] * v[3];
+        o[0] = gam * v[0] + gam * bv;
+        for (int a = 0; a < 3; ++a) o[a + 1] = gam * b[a] * v[0] + v[a + 1] + g1 * b[a] * bv;
+      }
+      pv = make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
+      const float4 rf = S.rf[i];
+      rv = make_float4(rf.x, rf.y, rf.z, (float)w);
+      have = true;
+    }
+  }
+  const unsigned long long m = __ballot(have);
+  unsigned long long base = 0;
+  if (lane == 0 && m) base = atomicAdd(count, (unsigned long long)__popcll(m));
+  base = __shfl(base, 0);
+  if (have) {
+    const int64_t o = out_base + (int64_t)base + __popcll(m & ((1ull << lane) - 1ull));
+    if (o < capacity) {
+      O.p[o] = pv;
+      O.r[o] = rv;
+      O.parent[o] = (int32_t)i;
+      O.meta[o] = TWO_BODY | 0x100;
+    }
+  }
+  double s = have ? w : 0.0;
+  for (int off = 32; off > 0; off >>= 1) s += shfl_f64(s, lane ^ off);
+  if (lane == 0 && s != 0.0) unsafeAtomicAdd(yield + 4, s);
+  wave_stat(stats, 1, have ? 1ull : 0ull);
+}
+
 // ------------------------------------------------------ m_V-continuous yield --
 constexpr int kYieldBatch = 16;
 constexpr int kYieldWindow = 4;  // trained masses a pass may use (consecutive)

@@ -47,6 +47,7 @@ enum CounterIdx {
   C_MUONS = 19,               // primaries that are muons (muons only come from primaries)
   C_DARK_N = 20,              // dark entries written in the current pass
   C_DARK_STATS = 21,          // [3] drate rows summing to zero, vectors written, bound-electron momentum failures
+  C_MESON_N = 24,             // meson-decay entries written in the current pass
   C_STATS = 32,               // [ST_N] run statistics
   C_PTRIALS = 64,             // [PETITE_NPROC] candidates per process
   C_PDRAWS = 80,              // [PETITE_NPROC] draws per process
@@ -1673,7 +1674,7 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
 static int check_dark_desc(const PetiteDarkDesc* desc, const double* blob, int64_t n, const char* who) {
   if (desc->n_active < 0 || desc->n_active > 8) return fail(PETITE_ERR_ARG, std::string(who) + ": n_active outside 0..8");
   for (int a = 0; a < desc->n_active; ++a)
-    if (desc->active[a] < DARKBREM || desc->active[a] > DARKMUONBREM)
+    if ((desc->active[a] < DARKBREM || desc->active[a] > DARKMUONBREM) && desc->active[a] != dark::TWO_BODY)
       return fail(PETITE_ERR_ARG, std::string(who) + ": active process is not a dark process");
   for (int f = 0; f < PETITE_DARK_FAMILIES; ++f) {
     if (desc->w_n[f] < 0 || (desc->w_n[f] > 0 && (desc->w_off[f] < 0 || desc->w_off[f] + 2 * (int64_t)desc->w_n[f] > n)))
@@ -1707,7 +1708,7 @@ int petite_dark_setup(PetiteCtx* c, const PetiteDarkDesc* desc, const double* bl
   }
   HIP_TRY(hipMalloc(&c->dark_blob, n * sizeof(double)));
   HIP_TRY(hipMemcpy(c->dark_blob, blob, n * sizeof(double), hipMemcpyHostToDevice));
-  if (!c->dyield) HIP_TRY(hipMalloc(&c->dyield, 4 * sizeof(double)));
+  if (!c->dyield) HIP_TRY(hipMalloc(&c->dyield, 5 * sizeof(double)));
   c->dark = *desc;
   c->has_dark = true;
   return PETITE_OK;
@@ -1731,9 +1732,10 @@ int petite_dark_reweight(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, uint
   rc = ensure_requests(c, chunk * na);
   if (rc) return rc;
   unsigned long long* ctr = c->counters;
-  HIP_TRY(hipMemsetAsync(c->dyield, 0, 4 * sizeof(double), s));
+  HIP_TRY(hipMemsetAsync(c->dyield, 0, 5 * sizeof(double), s));
   HIP_TRY(hipMemsetAsync(ctr, 0, C_N * sizeof(unsigned long long), s));
   int64_t out_base = 0;
+  const bool mesons = c->dark.meson_w[0] > 0.0 || c->dark.meson_w[1] > 0.0 || c->dark.meson_w[2] > 0.0;
   double t_sel = 0.0, t_dr = 0.0, t_em = 0.0;
   int64_t launches = 0;
   for (int64_t first = 0; first < n_sm; first += chunk) {
@@ -1793,11 +1795,25 @@ int petite_dark_reweight(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, uint
     }
     ++launches;
     out_base += (int64_t)n_e;
+    if (mesons) {
+      HIP_TRY(hipMemsetAsync(ctr + C_MESON_N, 0, sizeof(unsigned long long), s));
+      dark::EmitOut O{(float4*)out->p, (float4*)out->r, out->parent, out->meta};
+      hipLaunchKernelGGL(dark::k_meson_decay, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c->dev, c->dark, S,
+                         first, n, seed, O, out_base, out->capacity, ctr + C_MESON_N, c->dyield, ctr + C_DARK_STATS);
+      HIP_TRY(hipGetLastError());
+      unsigned long long n_m = 0;
+      HIP_TRY(hipMemcpyAsync(&n_m, ctr + C_MESON_N, sizeof(n_m), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      stats->n_entries += (int64_t)n_m;
+      if (out_base + (int64_t)n_m > out->capacity)
+        return fail(PETITE_ERR_CAPACITY, "petite_dark_reweight: output smaller than the entries (n_entries holds the count so far)");
+      out_base += (int64_t)n_m;
+    }
   }
   unsigned long long hs[ST_N], dst[3];
   HIP_TRY(hipMemcpyAsync(hs, ctr + C_STATS, sizeof(hs), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(dst, ctr + C_DARK_STATS, sizeof(dst), hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(stats->yield, c->dyield, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(stats->yield, c->dyield, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   stats->n_vectors = (int64_t)dst[1];
   stats->n_zero_rate = (int64_t)dst[0];

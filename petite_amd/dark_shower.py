@@ -16,13 +16,14 @@ import numpy as np
 import torch
 
 from .batch import ParticleBatch
-from .constants import all_process_code, alpha_em, dark_process_codes, mass_dict, PID_DARK_VECTOR
+from .constants import (TWO_BODY_CODE, PID_DARK_VECTOR, all_process_code, alpha_em, dark_process_codes, mass_dict,
+                        meson_twobody_branchingratios)
 from .dark_tables import DarkTables
 from .device import DeviceContext
 from .particle import Particle
 from .shower import Shower
 
-_DARK_NAMES = {8: "DarkBrem", 9: "DarkAnn", 10: "DarkComp", 11: "DarkMuonBrem"}
+_DARK_NAMES = {8: "DarkBrem", 9: "DarkAnn", 10: "DarkComp", 11: "DarkMuonBrem", TWO_BODY_CODE: "TwoBody_BSMDecay"}
 _MESONS = (111, 221, 331)
 
 
@@ -71,7 +72,7 @@ class DarkBatch:
     @property
     def yields(self):
         """Sum of vector weights per process (float64, accumulated on the device)."""
-        return dict(zip(("DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"), self.stats["yield"]))
+        return dict(zip(("DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem", "TwoBody_BSMDecay"), self.stats["yield"]))
 
     def order(self):
         """Indices of the valid entries in the reference's order: SM record
@@ -99,7 +100,8 @@ class DarkBatch:
             name = _DARK_NAMES[int(proc[e])]
             if self.bound_electron and name in ("DarkAnn", "DarkComp"):
                 name += "_bound"
-            ids = {"PID": PID_DARK_VECTOR, "parent_PID": par["PID"], "ID": 2 * par["ID"] + 0,
+            k = 1 if name == "TwoBody_BSMDecay" else 0  # dark_shower.py:838 vs 790
+            ids = {"PID": PID_DARK_VECTOR, "parent_PID": par["PID"], "ID": 2 * par["ID"] + k,
                    "parent_ID": par["ID"], "generation_number": par["generation_number"] + 1,
                    "generation_process": name, "weight": float(r[e, 3])}
             out.append(Particle(p[e], r[e, :3], ids))
@@ -165,7 +167,8 @@ class DarkShower(Shower):
         return self._dark.ns["DarkMuonBrem"](E)
 
     def _device_active(self):
-        return [all_process_code[p] for p in self.active_processes if p in all_process_code]
+        return [TWO_BODY_CODE if p == "TwoBody_BSMDecay" else all_process_code[p] for p in self.active_processes
+                if p in all_process_code or p == "TwoBody_BSMDecay"]
 
     def device_context(self, MS=True):
         key = bool(MS)
@@ -195,9 +198,13 @@ class DarkShower(Shower):
             PID, energy_initial = particle.get_ids()["PID"], particle.get_p0()[0]
         PID = int(PID)
         if PID in _MESONS:
-            if process == "TwoBody_BSMDecay":
-                raise NotImplementedError("meson two-body decays (TwoBody_BSMDecay) are not implemented")
-            return 0.0
+            if process != "TwoBody_BSMDecay":
+                return 0.0
+            m_meson = particle.get_ids()["mass"] if isinstance(particle, Particle) else mass_dict[PID]
+            r = self._mV / m_meson
+            if r >= 1.0:
+                return 0.0
+            return 2 * self.kinetic_mixing ** 2 * (1.0 - r ** 2) ** 3 * meson_twobody_branchingratios[PID]
         if process not in ("DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"):
             return 0.0
         return float(self._dark.bsm_weight(np.array([PID]), np.array([float(energy_initial)]), process)[0])
@@ -234,9 +241,6 @@ class DarkShower(Shower):
             store, n = record._store, record.n
         else:
             store, n = self._store_from_particles(record), len(record)
-            pids = set(int(v) for v in store["pid"].unique().cpu().numpy())
-            if pids & set(_MESONS) and "TwoBody_BSMDecay" in self.active_processes:
-                raise NotImplementedError("meson two-body decays (TwoBody_BSMDecay) are not implemented")
         ctx = self.device_context(MS=GlobalMS)
         out, n_e, stats = ctx.dark_reweight(store, n, self._seed if seed is None else seed)
         return DarkBatch(out, n_e, stats, ctx.dark_active, self.bound_electron)

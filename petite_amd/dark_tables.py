@@ -21,7 +21,8 @@ reference does.
 """
 import numpy as np
 
-from .constants import GeVsqcm2, alpha_em, cmtom, m_electron, m_muon
+from .constants import (TWO_BODY_CODE, GeVsqcm2, alpha_em, cmtom, m_electron, m_muon,
+                        meson_twobody_branchingratios)
 from .tables import MaterialTables, _load, geometric
 
 DARK_PROCESSES = ["DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"]
@@ -448,6 +449,10 @@ def build_dark_blob(dt: DarkTables, active_codes):
     for i, c in enumerate(codes):
         desc.active[i] = int(c)
     desc.eps2 = float(dt.eps2)
+    if TWO_BODY_CODE in codes:
+        # GetBSMWeights of a meson (dark_shower.py:633-640): 2 kappa^2 (1 - r^2)^3 BR(M -> gamma gamma)
+        for i, pid in enumerate((111, 221, 331)):
+            desc.meson_w[i] = 2.0 * float(dt.kinetic_mixing) ** 2 * meson_twobody_branchingratios[pid]
     desc.E_res, desc.E_C = float(dt.E_res), float(dt.E_C)
     desc.min_brem = float(dt.min_calc_dark[11]["DarkBrem"])
     desc.min_ann = float(dt.min_calc_dark[-11]["DarkAnn"])

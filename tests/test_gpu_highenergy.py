@@ -54,7 +54,8 @@ def test_muon_processes_and_dark_entries(run):
     dark = ds.generate_dark_showers(batch)
     codes = set(int(c) for c in dark.process[dark.valid].cpu().numpy())
     assert {8, 9, 10, 11} <= codes, codes
-    y = dark.yields
+    y = dict(dark.yields)
+    assert y.pop("TwoBody_BSMDecay", 0.0) == 0.0  # no meson records in an e/mu/gamma shower
     assert all(v > 0 for v in y.values()), y
 
 
