@@ -32,7 +32,8 @@ for _tm in target_information:
     target_information[_tm]["dEdx"] = 2.0 * target_information[_tm]["rho"]  # MeV/cm
 
 mass_dict = {11: m_electron, -11: m_electron, 22: 0.0, 13: m_muon, -13: m_muon,
-             12: 0.0, -12: 0.0, 14: 0.0, -14: 0.0}
+             12: 0.0, -12: 0.0, 14: 0.0, -14: 0.0,
+             111: 134.9768e-3, 221: 547.862e-3, 331: 957.78e-3}  # physical_constants.py:31-36
 
 # process codes (src/PETITE/shower.py:36) and dark extensions
 PROCESS_CODE = {"Brem": 0, "Ann": 1, "PairProd": 2, "Comp": 3, "Moller": 4, "Bhabha": 5,

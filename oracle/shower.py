@@ -11,6 +11,10 @@ Philox stream (oracle.philox), children's stream ids derive from the parent
 stream, so the reference's FIFO sweep order and any other order give the same
 shower.
 
+Other species (mesons, neutrinos) end where they start: the reference
+propagates none of them and its meson decay returns no products
+(shower.py:640-653, particle.py:391-423).
+
 Counting convention: one particle-step is one charged substep (shower.py
 559-581, the one that ends in a hard interaction included), one closing
 increment (583-598) or one photon flight (538-553).
@@ -26,6 +30,9 @@ from . import philox, vegas
 from .constants import PROCESS_PIDS, cmtom, mass_dict, m_electron
 from .kinematics import sm_final_state
 from .moliere import scatter
+
+
+PROPAGATED = (22, 11, -11, 13, -13)
 
 
 class OParticle:
@@ -108,11 +115,17 @@ class OracleShower:
         return self.maxf[key]
 
     def pme(self, p):
-        return max(self.T.min_calc[p.pid], self.min_energy, p.mass)
+        return max(self.T.min_calc.get(p.pid, 0.0), self.min_energy, p.mass)
 
     # -- transport ------------------------------------------------------------
     def propagate(self, p):
         T, seed, st = self.T, self.seed, self.stats
+        if p.pid not in PROPAGATED:
+            # shower.py:640-653 propagates photons, e+- and mu+- only; a
+            # neutrino is ended where it is, and a decaying meson's
+            # decay_particle (particle.py:391-423) returns no products
+            p.ended = True
+            return
         pme = self.pme(p)
         if p.p0[0] < pme:
             p.ended = True

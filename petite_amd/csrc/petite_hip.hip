@@ -616,7 +616,9 @@ __device__ inline void charged_advance(const DevTables& T, double dist, double m
   const double Eb = E;
   E = E - T.d->dEdx * dist;
   if (E <= m) E = m;
-  p = sqrt(E * E - m * m);
+  // fmax: contracted to fma(E, E, -(m*m)), E*E - m*m at E == m is minus the
+  // rounding error of m*m, which can be negative (the reference's p is 0 there)
+  p = sqrt(fmax(E * E - m * m, 0.0));
   dep += Eb - E;
   if (p > 0.0) {
     rx += (float)(nx * dist);
@@ -696,15 +698,18 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
             ny *= inv;
             nz *= inv;
           }
-          p = sqrt(fmax(E * E - m * m, 0.0));
+          // a species the shower does not propagate (mesons, neutrinos: no
+          // channels and not a photon) keeps its |p|, and ends where it starts
+          const bool still = ch == 0xFFFu && pid != 22;
+          p = still ? sqrt((double)pn2) : sqrt(fmax(E * E - m * m, 0.0));
           rx = r0.x;
           ry = r0.y;
           rz = r0.z;
           dep = 0.0;
           k = 0;
           closing = false;
-          if (E < pme) {
-            wait = true;  // ends where it starts (shower.py:532-536)
+          if (E < pme || still) {
+            wait = true;  // ends where it starts (shower.py:532-536, 640-653)
           } else if (pid == 22) {
             // one exponential flight, no MCS (shower.py:538-553)
             const double mfp = L.mfp(22, E);

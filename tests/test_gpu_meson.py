@@ -81,3 +81,65 @@ def test_drop_in_list_ids_and_other_processes():
         assert d["ID"] == 2 * d["parent_ID"] + 1 and d["parent_PID"] in (111, 221, 331)
     # the electrons' dark-brem entries are still there
     assert any(v.get_ids()["generation_process"] == "DarkBrem" for v in vecs)
+
+
+def _record(pid, E, sid, rng):
+    from petite_amd.batch import PRIMARY_DTYPE
+    m = mass_dict[pid]
+    th, ph = abs(rng.normal(0, 0.05)), rng.uniform(0, 2 * np.pi)
+    p = math.sqrt(E * E - m * m)
+    r = np.zeros(1, dtype=PRIMARY_DTYPE)
+    r["E"], r["px"], r["py"], r["pz"] = E, p * math.sin(th) * math.cos(ph), p * math.sin(th) * math.sin(ph), p * math.cos(th)
+    r["x"], r["y"], r["z"] = rng.normal(0, 0.01, 3)
+    r["mass"], r["weight"], r["pid"], r["shower"], r["sid"] = m, 1.0, pid, sid, sid
+    return r
+
+
+def test_meson_and_neutrino_primaries_end_where_they_start():
+    # shower.py:640-653 propagates photons, e+- and mu+- only; a meson's
+    # decay_particle returns no products (particle.py:391-423): a meson or
+    # neutrino primary is its shower's only record, with pf = p0 and rf = r0,
+    # and the other primaries' showers are unchanged bit for bit
+    from petite_amd import Shower
+    sh = Shower("100GeV", "graphite", 0.010, seed=5)
+    rng = np.random.default_rng(9)
+    base = np.concatenate([_record(p, E, i, rng) for i, (p, E) in enumerate([(11, 3.0), (22, 2.0), (-11, 1.5)])])
+    extra = np.concatenate([_record(p, E, 3 + i, rng)
+                            for i, (p, E) in enumerate([(111, 5.0), (221, 2.0), (331, 40.0), (14, 3.0), (-14, 0.5)])])
+    a = sh.generate_showers(base, seed=5).to_numpy()
+    mixed = sh.generate_showers(np.concatenate([base, extra]), seed=5)
+    b = mixed.to_numpy()
+    ia = np.argsort(a["sid"], kind="stable")
+    sel = np.nonzero(b["shower"] < 3)[0]
+    ib = sel[np.argsort(b["sid"][sel], kind="stable")]
+    assert len(ia) == len(ib)
+    # a lepton stopped at rest has pf = (m, 0, 0, 0), as lose_energy sets it (particle.py:143-153)
+    assert np.all(np.isfinite(b["pf"])) and np.all(np.isfinite(a["pf"]))
+    rest = a["pf"][:, 0] == np.float32(mass_dict[11])
+    assert np.all(a["pf"][rest & (np.abs(a["pid"]) == 11), 1:] == 0.0)
+    for f in ("p0", "r0", "pf", "rf", "pid", "sid", "meta"):
+        assert np.array_equal(a[f][ia], b[f][ib]), f
+    e = mixed.energy.cpu().numpy()
+    for r in extra:
+        s = int(r["shower"])
+        (i,) = np.nonzero(b["shower"] == s)[0]
+        assert b["pid"][i] == r["pid"]
+        np.testing.assert_allclose(b["pf"][i], b["p0"][i], rtol=1e-6, atol=1e-7 * r["E"])
+        np.testing.assert_array_equal(b["rf"][i][:3], b["r0"][i][:3])
+        assert b["rf"][i][3] == 0  # no steps
+        np.testing.assert_allclose(e[s], [0.0, r["E"], 0.0, 0.0], rtol=1e-6)
+
+
+def test_meson_particle_through_the_list_api():
+    # SParams path: the SM list is the meson alone, the vector list its
+    # TwoBody_BSMDecay vector with ID 2 ID + 1 (dark_shower.py:826-844)
+    ds = DarkShower("100GeV", "graphite", 0.010, 0.03, seed=SEED)
+    m = mass_dict[221]
+    q = Particle([10.0, 0.0, 0.0, math.sqrt(100.0 - m * m)], [0.0, 0.0, 0.1], {"PID": 221, "ID": 4, "weight": 2.0})
+    sm, vecs = ds.generate_dark_shower(SParams=q)
+    assert len(sm) == 1 and sm[0].get_ids()["PID"] == 221
+    assert len(vecs) == 1
+    d = vecs[0].get_ids()
+    assert d["generation_process"] == "TwoBody_BSMDecay" and d["ID"] == 9 and d["parent_PID"] == 221
+    np.testing.assert_allclose(d["weight"], 2.0 * ds.GetBSMWeights(q, "TwoBody_BSMDecay"), rtol=1e-6)
+    np.testing.assert_allclose(vecs[0].get_rf(), [0.0, 0.0, 0.1], atol=1e-7)

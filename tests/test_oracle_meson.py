@@ -74,3 +74,19 @@ def test_produce_uses_the_dark_stream():
     assert np.array_equal(a["p"], b["p"]) and not np.array_equal(a["p"], c["p"])
     np.testing.assert_allclose(a["weight"], 2.0 * od.meson_weight(111, od.MESON_MASS[111], 0.02), rtol=1e-15)
     assert od.produce_meson(rec, 0.2, seed=5) is None
+
+
+def test_mesons_and_neutrinos_end_where_they_start():
+    # generate_shower propagates photons, e+- and mu+- only (shower.py:640-653);
+    # a neutrino is ended in place and a meson's decay_particle returns no
+    # products (particle.py:391-423), so the shower is the primary alone
+    from oracle.shower import OracleShower
+    from oracle.tables import SMTables
+    sh = OracleShower(SMTables("graphite"), 0.010, seed=3)
+    for s, pid in enumerate((111, 221, 331, 14, -14)):
+        m = od.MESON_MASS.get(pid, 0.0)
+        p0 = _pf(7.0, m, [0.1, -0.2, 1.0])
+        out = sh.generate_shower(pid, p0, r0=(0.01, 0.02, 0.3), sid=s, shower=s)
+        assert len(out) == 1 and out[0].ended and out[0].steps == 0
+        assert np.array_equal(out[0].pf, out[0].p0) and np.array_equal(out[0].rf, out[0].r0)
+        assert sh.stats.final[s] == 7.0 and s not in sh.stats.deposit
