@@ -196,6 +196,9 @@ def main():
     train_out = bench_map_training(cpu=not args.no_cpu_baseline)
     print(f"map training: {train_out['samples']} samples, {train_out['s_total']:.2f} s", file=sys.stderr, flush=True)
 
+    meson_out = bench_meson_beam(cpu=not args.no_cpu_baseline)
+    print(f"meson beam: {meson_out['records']} records, {meson_out['ms']:.3f} ms", file=sys.stderr, flush=True)
+
     dark_out = yield_out = None
     if args.dark_forward:
         del sh, ctx
@@ -244,6 +247,7 @@ def main():
         out["gpu_over_cpu_showers"] = showers_per_s / cpu["showers_per_s"]
     out["detector_cut"] = det_out
     out["map_training"] = train_out
+    out["meson_beam"] = meson_out
     if dark_out:
         out["dark_forward"] = dark_out
     if yield_out:
@@ -289,6 +293,52 @@ def bench_detector(batch, reps=10, cpu=True):
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": m / dt, "unit": "records/s", "cores": 1, "kind": "port",
                                "sample": f"first {m} records, {dt:.2f} s (oracle numpy float64)"}
+    return out
+
+
+def bench_meson_beam(n=4_000_000, reps=5, cpu=True):
+    """TwoBody_BSMDecay of a meson beam (dark_shower.py:633-640, 836-844): a
+    synthetic 3:1 pi0/eta beam of (E, px, py, pz) rows, E log-uniform in
+    [0.2, 120] GeV within 20 mrad, as a device record (ParticleBatch.from_arrays),
+    m_V = 30 MeV; host-timed generate_dark_showers calls (launches, the
+    count readback and the yield sums included)."""
+    import torch
+
+    from petite_amd.batch import ParticleBatch
+    from petite_amd.constants import mass_dict
+    from petite_amd.dark_shower import DarkShower
+    rng = np.random.default_rng(SEED)
+    pid = np.where(rng.uniform(size=n) < 0.75, 111, 221).astype(np.int32)
+    m = np.where(pid == 111, mass_dict[111], mass_dict[221])
+    E = np.exp(rng.uniform(np.log(0.2), np.log(120.0), n))
+    E = np.maximum(E, 1.001 * m)
+    th, ph = np.abs(rng.normal(0, 0.02, n)), rng.uniform(0, 2 * np.pi, n)
+    p = np.sqrt(E * E - m * m)
+    beam = np.stack([E, p * np.sin(th) * np.cos(ph), p * np.sin(th) * np.sin(ph), p * np.cos(th)], axis=1)
+    ds = DarkShower("100GeV", "graphite", CUTOFF, 0.03, seed=SEED, active_processes=["TwoBody_BSMDecay"])
+    rec = ParticleBatch.from_arrays(beam, pid)
+    dark = ds.generate_dark_showers(rec)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dark = ds.generate_dark_showers(rec)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    out = {"workload": f"{n} mesons (3:1 pi0/eta), E log-uniform [0.2, 120] GeV, m_V = 30 MeV, TwoBody_BSMDecay only",
+           "records": n, "vectors": int(dark.n), "ms": ms, "records_per_s": n / (ms / 1e3),
+           "yield": float(dark.yields["TwoBody_BSMDecay"])}
+    if cpu:
+        from oracle import dark as od
+        k = 20_000
+        a = rec.to_numpy()
+        t0 = time.perf_counter()
+        for i in range(k):
+            od.produce_meson({"pid": int(a["pid"][i]), "pf": a["pf"][i].astype(np.float64), "rf": a["rf"][i, :3],
+                              "weight": float(a["r0"][i, 3]), "sid": np.uint64(i)}, ds.get_mV(), SEED)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": k / dt, "unit": "records/s", "cores": 1, "kind": "port",
+                               "sample": f"first {k} records, {dt:.2f} s (oracle produce_meson, float64 scalar)"}
+    del dark, rec
     return out
 
 

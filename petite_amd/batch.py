@@ -100,6 +100,34 @@ class ParticleBatch:
         np.savez(path, petite_record=np.int32(1), n=np.int64(self.n), stats=np.array(json.dumps(self.stats)), **arrays)
 
     @classmethod
+    def from_arrays(cls, p4, pid, r=None, weight=None, device="cuda"):
+        """Record of N untransported particles, e.g. a meson beam of
+        (E, px, py, pz) rows (examples/beams/*.npy), for
+        ``DarkShower.generate_dark_showers``: pf = p0, rf = r0, record i is
+        shower i with stream id i, as a list of N Particles would be."""
+        p4 = np.asarray(p4, dtype=np.float64)
+        if p4.ndim != 2 or p4.shape[1] != 4:
+            raise ValueError(f"from_arrays: p4 must have shape (N, 4), got {p4.shape}")
+        n = p4.shape[0]
+        pid = np.broadcast_to(np.asarray(pid, dtype=np.int32), (n,))
+        r = np.zeros((n, 3)) if r is None else np.asarray(r, dtype=np.float64)
+        if r.shape != (n, 3):
+            raise ValueError(f"from_arrays: r must have shape ({n}, 3), got {r.shape}")
+        w = np.broadcast_to(np.asarray(1.0 if weight is None else weight, dtype=np.float64), (n,))
+        dev = torch.device(device)
+        p0 = torch.from_numpy(p4.astype(np.float32)).to(dev)
+        r0 = torch.from_numpy(np.concatenate([r, w[:, None]], axis=1).astype(np.float32)).to(dev)
+        rf = r0.clone()
+        rf[:, 3] = 0.0  # no steps
+        idx = torch.arange(n, device=dev, dtype=torch.int32)
+        store = {"p0": p0, "pf": p0.clone(), "r0": r0, "rf": rf,
+                 "pid": torch.from_numpy(np.ascontiguousarray(pid)).to(dev),
+                 "parent": torch.full((n,), -1, device=dev, dtype=torch.int32),
+                 "meta": torch.zeros(n, device=dev, dtype=torch.int32),
+                 "shower": idx, "sid": idx.to(torch.int64)}
+        return cls(store, n)
+
+    @classmethod
     def load(cls, path, device="cuda"):
         """Record written by ``save`` (read with allow_pickle=False)."""
         with np.load(path, allow_pickle=False) as z:

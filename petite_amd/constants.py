@@ -61,3 +61,18 @@ PID_DARK_VECTOR = 4900022
 # of the mesons it applies to (particle.py:30-48)
 TWO_BODY_CODE = 12
 meson_twobody_branchingratios = {111: 0.98823, 221: 0.3936, 331: 0.02307}
+
+# meson decays (particle.py:15-47, physical_constants.py:31-33): proper decay
+# lengths c tau [m], interaction lengths of the long-lived mesons [m], and the
+# decay channels as [branching ratio, product PIDs]
+c_tau_pi0, c_tau_pi_pm, c_tau_K_pm = 25.3e-9, 7.8045, 3.711
+int_length_dict = {211: 1.796e-1, -211: 1.796e-1, 321: 2.2875e-1, -321: 2.2875e-1}
+decay_length_dict = {211: c_tau_pi_pm, -211: c_tau_pi_pm, 321: c_tau_K_pm, -321: c_tau_K_pm, 111: c_tau_pi0}
+meson_decay_dict = {111: [[0.98823, [22, 22]]],
+                    221: [[0.3936, [22, 22]], [0.3257, [111, 111, 111]]],
+                    331: [[0.02307, [22, 22]], [0.224, [111, 111, 221]], [0.00250, [111, 111, 111]]],
+                    223: [[0.0828, [22, 111]]],
+                    211: [[0.9998, [-13, 14]]],
+                    -211: [[0.9998, [13, -14]]],
+                    321: [[0.6356, [-13, 14]]],
+                    -321: [[0.6356, [13, -14]]]}

@@ -143,3 +143,30 @@ def test_meson_particle_through_the_list_api():
     assert d["generation_process"] == "TwoBody_BSMDecay" and d["ID"] == 9 and d["parent_PID"] == 221
     np.testing.assert_allclose(d["weight"], 2.0 * ds.GetBSMWeights(q, "TwoBody_BSMDecay"), rtol=1e-6)
     np.testing.assert_allclose(vecs[0].get_rf(), [0.0, 0.0, 0.1], atol=1e-7)
+
+
+def test_meson_beam_from_arrays_matches_the_list_path():
+    # a meson beam of (E, px, py, pz) rows (examples/beams/*.npy) as a device
+    # record: the same vectors as the list of Particles, record i = list entry i
+    from petite_amd.batch import ParticleBatch
+    ds = DarkShower("100GeV", "graphite", 0.010, 0.03, seed=SEED, active_processes=["TwoBody_BSMDecay"])
+    rng = np.random.default_rng(12)
+    n = 3000
+    m = mass_dict[111]
+    E = np.exp(rng.uniform(np.log(0.2), np.log(120.0), n))
+    th = np.abs(rng.normal(0, 0.02, n))
+    ph = rng.uniform(0, 2 * np.pi, n)
+    p = np.sqrt(E * E - m * m)
+    beam = np.stack([E, p * np.sin(th) * np.cos(ph), p * np.sin(th) * np.sin(ph), p * np.cos(th)], axis=1)
+    rec = ParticleBatch.from_arrays(beam, 111)
+    d_arr = ds.generate_dark_showers(rec)
+    parts = [Particle(list(beam[i]), [0.0, 0.0, 0.0], {"PID": 111, "mass": m}) for i in range(n)]
+    d_list = ds.generate_dark_showers(parts)
+    assert d_arr.n == d_list.n == n
+    oa = np.argsort(d_arr.parent.cpu().numpy())
+    ol = np.argsort(d_list.parent.cpu().numpy())
+    for k in ("p", "r"):
+        assert torch.equal(d_arr.tensor(k)[oa], d_list.tensor(k)[ol]), k
+    np.testing.assert_allclose(d_arr.yields["TwoBody_BSMDecay"], d_list.yields["TwoBody_BSMDecay"], rtol=1e-12)
+    with pytest.raises(ValueError):
+        ParticleBatch.from_arrays(beam[:, :3], 111)
