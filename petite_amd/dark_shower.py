@@ -32,12 +32,13 @@ class DarkBatch:
     with a positive weight; ``valid`` marks the entries that hold a vector
     (produce_bsm_particle returned a Particle)."""
 
-    def __init__(self, out, n, stats, active, bound_electron=True):
+    def __init__(self, out, n, stats, active, bound_electron=True, mV=None):
         self.n = int(n)
         self._out = out
         self.stats = stats
         self.active = list(active)
         self.bound_electron = bound_electron
+        self.mV = mV
 
     def __len__(self):
         return self.n
@@ -104,6 +105,11 @@ class DarkBatch:
             ids = {"PID": PID_DARK_VECTOR, "parent_PID": par["PID"], "ID": 2 * par["ID"] + k,
                    "parent_ID": par["ID"], "generation_number": par["generation_number"] + 1,
                    "generation_process": name, "weight": float(r[e, 3])}
+            if self.mV is not None:
+                # TwoBody_BSMDecay sets m_V (dark_shower.py:839); produce_bsm_particle sets none and
+                # Particle infers it from the float64 vector, rounded to 6 decimals (particle.py:125-131),
+                # which the float32 vector here cannot reproduce
+                ids["mass"] = float(self.mV) if k else round(float(self.mV), 6)
             out.append(Particle(p[e], r[e, :3], ids))
         return out
 
@@ -243,7 +249,7 @@ class DarkShower(Shower):
             store, n = self._store_from_particles(record), len(record)
         ctx = self.device_context(MS=GlobalMS)
         out, n_e, stats = ctx.dark_reweight(store, n, self._seed if seed is None else seed)
-        return DarkBatch(out, n_e, stats, ctx.dark_active, self.bound_electron)
+        return DarkBatch(out, n_e, stats, ctx.dark_active, self.bound_electron, self._mV)
 
     def generate_dark_shower(self, ExDir=None, SParams=None):
         """(SM shower, dark vectors) of an existing shower or of a new one from

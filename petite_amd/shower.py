@@ -27,6 +27,9 @@ from .particle import Particle
 from .tables import MaterialTables, table_set_for
 
 _CHARGED = (11, -11, 13, -13)
+# cross-section table whose energies a dark map is trained on: the maps of the
+# bound-electron processes were made on the free-electron energies
+_DARK_SIGMA_TABLE = {"DarkAnn": "DarkAnn_free", "DarkComp": "DarkComp_free"}
 
 
 def _seed_from(seed):
@@ -203,16 +206,25 @@ class Shower:
         max_F (petite_amd.vegas_train).  The reference ships its maps trained
         with the vegas package (utilities/generate_integrators.py); a retrained
         map keeps the end points and changes only the draws' efficiency.
-        energies="sigma" first puts each SM map on its process's cross-section
+        energies="sigma" first puts each map on its process's cross-section
         energies (e.g. up to 400 GeV in the 400 GeV set, whose Brem/PairProd/
-        MuonBrem maps are missing); an array puts every map on those energies."""
+        MuonBrem/DarkBrem/DarkMuonBrem maps are missing; DarkAnn and DarkComp
+        take their free-electron tables' energies); an array puts every map on
+        those energies."""
+        from .tables import regrid
         from .vegas_train import TRAIN_SEED
         if energies is not None:
             if processes is None or (isinstance(energies, str) and energies != "sigma"):
                 raise ValueError('train_maps: energies needs the processes, and takes "sigma" or an array')
+            dark = getattr(self, "_dark", None)
             for P in processes:
-                E_new = self._tables.nsigma_tab[P][0] if isinstance(energies, str) else energies
-                self._tables.regrid_map(P, E_new)
+                if P in self._tables.grids:
+                    grids, E_sig = self._tables.grids, lambda: self._tables.nsigma_tab[P][0]
+                elif dark is not None and P in dark.grids:
+                    grids, E_sig = dark.grids, lambda: dark.xsec[_DARK_SIGMA_TABLE.get(P, P)][:, 0]
+                else:
+                    raise ValueError(f"train_maps: {P} has no map")
+                regrid(grids, P, E_sig() if isinstance(energies, str) else energies)
             for c in self._ctx.values():
                 c.close()
             self._ctx = {}

@@ -152,21 +152,26 @@ class MaterialTables:
                              np.array(src[f"nodecnt__{P}"]))
 
     def regrid_map(self, process, E_new):
-        """Put the process's map on the energies E_new, each new energy taking
-        the nodes of the nearest old one in log E: the starting point for
-        retraining (Shower.train_maps) at energies the shipped maps lack."""
-        E, nodes, cnt = self.grids[process]
-        E_new = np.asarray(E_new, dtype=np.float64)
-        cnt = np.asarray(cnt)
-        stride = int(cnt[0].sum())
-        nodes = np.asarray(nodes, dtype=np.float32).reshape(len(E), stride)
-        idx = np.abs(np.log(E_new)[:, None] - np.log(np.asarray(E, dtype=np.float64))[None, :]).argmin(axis=1)
-        self.grids[process] = (E_new, nodes[idx].ravel().copy(), np.repeat(cnt[:1], len(E_new), axis=0))
+        regrid(self.grids, process, E_new)
 
     def nsigma(self, process, E):
         xg, yg = self.nsigma_tab[process]
         E = np.asarray(E, dtype=np.float64)
         return np.where((E < xg[0]) | (E > xg[-1]), 0.0, np.interp(E, xg, yg))
+
+
+def regrid(grids, process, E_new):
+    """Put a process's map (grids[process] = (E, nodes, counts)) on the
+    energies E_new, each new energy taking the nodes of the nearest old one in
+    log E: the starting point for retraining (Shower.train_maps) at energies
+    the shipped maps lack."""
+    E, nodes, cnt = grids[process]
+    E_new = np.asarray(E_new, dtype=np.float64)
+    cnt = np.asarray(cnt)
+    stride = int(cnt[0].sum())
+    nodes = np.asarray(nodes, dtype=np.float32).reshape(len(E), stride)
+    idx = np.abs(np.log(E_new)[:, None] - np.log(np.asarray(E, dtype=np.float64))[None, :]).argmin(axis=1)
+    grids[process] = (E_new, nodes[idx].ravel().copy(), np.repeat(cnt[:1], len(E_new), axis=0))
 
 
 def geometric(E):

@@ -81,6 +81,10 @@ def test_drop_in_list_ids_and_other_processes():
         assert d["ID"] == 2 * d["parent_ID"] + 1 and d["parent_PID"] in (111, 221, 331)
     # the electrons' dark-brem entries are still there
     assert any(v.get_ids()["generation_process"] == "DarkBrem" for v in vecs)
+    # vector masses: m_V (TwoBody_BSMDecay) or m_V rounded to 6 decimals, as the
+    # reference's Particle infers it from a float64 vector
+    for v in vecs:
+        assert abs(v.get_ids()["mass"] - ds.get_mV()) <= 5e-7 and v._mass == v.get_ids()["mass"]
 
 
 def _record(pid, E, sid, rng):

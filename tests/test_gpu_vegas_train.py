@@ -125,3 +125,27 @@ def test_missing_400gev_map_trained_at_its_energies():
     np.testing.assert_allclose(np.median(ratio), 1.0, atol=0.05)
     with pytest.raises(ValueError):
         sh.train_maps(None, energies="sigma")
+
+
+def test_missing_400gev_dark_brem_map_trained_at_its_energies():
+    # the 400 GeV set lacks the DarkBrem map too: the 100 GeV map of the same
+    # mass stands in; put on the 400 GeV dark cross-section energies and
+    # trained, E[J f] above 100 GeV reproduces the shipped DarkBrem table
+    from petite_amd.dark_shower import DarkShower
+    ds = DarkShower("400GeV", "lead", 0.100, 0.03, seed=5)
+    assert ds.device_context().tables_grid("DarkBrem")[0][-1] <= 100.0 + 1e-9
+    ds.train_maps(["DarkBrem"], energies="sigma", n_iter=5, n_per_energy=1 << 14)
+    ctx = ds.device_context()
+    E = ctx.tables_grid("DarkBrem")[0]
+    xs = ds._dark.xsec["DarkBrem"]
+    assert len(E) == len(xs) and E[-1] > 399.0
+    assert int(ctx.desc.grid[all_process_code["DarkBrem"]].n_E) == len(E)
+    hi = E > 100.0
+    ratio = ctx.sigma_est["DarkBrem"][hi] / xs[hi, 1]
+    assert hi.sum() >= 10
+    np.testing.assert_allclose(np.median(ratio), 1.0, atol=0.05)
+    # the retrained context still makes dark vectors
+    m = 0.51099895e-3
+    from petite_amd import Particle as P_
+    sm, vecs = ds.generate_dark_shower(SParams=P_([200.0, 0.0, 0.0, np.sqrt(200.0 ** 2 - m * m)], [0, 0, 0], {"PID": 11}))
+    assert any(v.get_ids()["generation_process"] == "DarkBrem" for v in vecs)
