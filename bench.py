@@ -301,7 +301,9 @@ def bench_meson_beam(n=4_000_000, reps=5, cpu=True):
     synthetic 3:1 pi0/eta beam of (E, px, py, pz) rows, E log-uniform in
     [0.2, 120] GeV within 20 mrad, as a device record (ParticleBatch.from_arrays),
     m_V = 30 MeV; host-timed generate_dark_showers calls (launches, the
-    count readback and the yield sums included)."""
+    count readback and the yield sums included), and k_meson_decay's HIP-event
+    time: 100 algorithmic bytes per meson record (pid, r0, sid, pf, rf read;
+    p, r, parent, meta written), 4 per other record (pid)."""
     import torch
 
     from petite_amd.batch import ParticleBatch
@@ -318,14 +320,25 @@ def bench_meson_beam(n=4_000_000, reps=5, cpu=True):
     ds = DarkShower("100GeV", "graphite", CUTOFF, 0.03, seed=SEED, active_processes=["TwoBody_BSMDecay"])
     rec = ParticleBatch.from_arrays(beam, pid)
     dark = ds.generate_dark_showers(rec)
+    ds.device_context().set_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    ms_kernel = 0.0
     for _ in range(reps):
         dark = ds.generate_dark_showers(rec)
+        ms_kernel += dark.stats["ms_meson"]
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) / reps * 1e3
+    ms_kernel /= reps
+    ds.device_context().set_timing(False)
+    nv = int(dark.n)
+    ach = (100.0 * nv + 4.0 * (n - nv)) / (ms_kernel / 1e3) / 1e9
     out = {"workload": f"{n} mesons (3:1 pi0/eta), E log-uniform [0.2, 120] GeV, m_V = 30 MeV, TwoBody_BSMDecay only",
-           "records": n, "vectors": int(dark.n), "ms": ms, "records_per_s": n / (ms / 1e3),
+           "records": n, "vectors": nv, "ms": ms, "records_per_s": n / (ms / 1e3),
+           "kernel_ms": ms_kernel,
+           "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                        "traffic": None, "kernel": "k_meson_decay",
+                        "per_unit": "100 B per meson record, 4 B per other record"},
            "yield": float(dark.yields["TwoBody_BSMDecay"])}
     if cpu:
         from oracle import dark as od

@@ -221,6 +221,7 @@ typedef struct {
                          * TwoBody_BSMDecay) */
   double ms_select, ms_draw, ms_emit;
   int64_t launches;
+  double ms_meson;      /* k_meson_decay time (HIP events; with petite_set_timing on) */
 } PetiteDarkStats;
 
 /* Attach the dark tables to a context whose table descriptor carries the dark

@@ -85,7 +85,7 @@ class DarkStats(C.Structure):
     _fields_ = [("n_entries", C.c_int64), ("n_vectors", C.c_int64), ("n_draws", C.c_int64), ("n_trials", C.c_int64),
                 ("n_no_sample", C.c_int64), ("n_zero_rate", C.c_int64), ("n_pe_fail", C.c_int64), ("pad", C.c_int64),
                 ("yield_", C.c_double * 5), ("ms_select", C.c_double), ("ms_draw", C.c_double),
-                ("ms_emit", C.c_double), ("launches", C.c_int64)]
+                ("ms_emit", C.c_double), ("launches", C.c_int64), ("ms_meson", C.c_double)]
 
     def as_dict(self):
         d = {f: (float(getattr(self, f)) if t is C.c_double else int(getattr(self, f)))

@@ -109,7 +109,7 @@ class ParticleBatch:
         if p4.ndim != 2 or p4.shape[1] != 4:
             raise ValueError(f"from_arrays: p4 must have shape (N, 4), got {p4.shape}")
         n = p4.shape[0]
-        pid = np.broadcast_to(np.asarray(pid, dtype=np.int32), (n,))
+        pid = np.array(np.broadcast_to(np.asarray(pid, dtype=np.int32), (n,)))  # a writable copy for torch
         r = np.zeros((n, 3)) if r is None else np.asarray(r, dtype=np.float64)
         if r.shape != (n, 3):
             raise ValueError(f"from_arrays: r must have shape ({n}, 3), got {r.shape}")
@@ -121,7 +121,7 @@ class ParticleBatch:
         rf[:, 3] = 0.0  # no steps
         idx = torch.arange(n, device=dev, dtype=torch.int32)
         store = {"p0": p0, "pf": p0.clone(), "r0": r0, "rf": rf,
-                 "pid": torch.from_numpy(np.ascontiguousarray(pid)).to(dev),
+                 "pid": torch.from_numpy(pid).to(dev),
                  "parent": torch.full((n,), -1, device=dev, dtype=torch.int32),
                  "meta": torch.zeros(n, device=dev, dtype=torch.int32),
                  "shower": idx, "sid": idx.to(torch.int64)}

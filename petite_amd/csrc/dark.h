@@ -462,12 +462,20 @@ constexpr int TWO_BODY = 12;  // TwoBody_BSMDecay: dark process code of the outp
 // vector sits at the meson's end point.  Meson masses are the PDG values of
 // mass_dict.  Entries are appended after out_base through *count; an entry
 // beyond capacity is counted but not written.
-__global__ __launch_bounds__(256) void k_meson_decay(DevTables T, PetiteDarkDesc D, DevStore S, int64_t first,
-                                                     int64_t n, uint64_t seed, EmitOut O, int64_t out_base,
-                                                     int64_t capacity, unsigned long long* count, double* yield,
-                                                     unsigned long long* stats) {
+// Slots, the yield sum and the vector count are aggregated per block (one
+// atomic of each per 1024 records): per-wave atomics on the three shared
+// addresses serialised the kernel (845 us for 4e6 records).
+constexpr int kMesonBlock = 1024;
+__global__ __launch_bounds__(kMesonBlock) void k_meson_decay(DevTables T, PetiteDarkDesc D, DevStore S, int64_t first,
+                                                             int64_t n, uint64_t seed, EmitOut O, int64_t out_base,
+                                                             int64_t capacity, unsigned long long* count,
+                                                             double* yield, unsigned long long* stats) {
+  __shared__ unsigned int wave_off[kMesonBlock / 64];
+  __shared__ double wave_w[kMesonBlock / 64];
+  __shared__ unsigned long long block_base;
   const int64_t i = first + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
   bool have = false;
   double w = 0.0;
   float4 pv = make_float4(0.f, 0.f, 0.f, 0.f), rv = pv;
@@ -511,11 +519,31 @@ __global__ __launch_bounds__(256) void k_meson_decay(DevTables T, PetiteDarkDesc
     }
   }
   const unsigned long long m = __ballot(have);
-  unsigned long long base = 0;
-  if (lane == 0 && m) base = atomicAdd(count, (unsigned long long)__popcll(m));
-  base = __shfl(base, 0);
+  double s = have ? w : 0.0;
+  for (int off = 32; off > 0; off >>= 1) s += shfl_f64(s, lane ^ off);
+  if (lane == 0) {
+    wave_off[wave] = (unsigned int)__popcll(m);
+    wave_w[wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int tot = 0;
+    double sw = 0.0;
+    for (int k = 0; k < kMesonBlock / 64; ++k) {
+      const unsigned int c = wave_off[k];
+      wave_off[k] = tot;  // exclusive prefix: the wave's first slot in the block
+      tot += c;
+      sw += wave_w[k];
+    }
+    block_base = tot ? atomicAdd(count, (unsigned long long)tot) : 0ull;
+    if (tot) {
+      unsafeAtomicAdd(yield + 4, sw);
+      atomicAdd(stats + 1, (unsigned long long)tot);
+    }
+  }
+  __syncthreads();
   if (have) {
-    const int64_t o = out_base + (int64_t)base + __popcll(m & ((1ull << lane) - 1ull));
+    const int64_t o = out_base + (int64_t)(block_base + wave_off[wave]) + __popcll(m & ((1ull << lane) - 1ull));
     if (o < capacity) {
       O.p[o] = pv;
       O.r[o] = rv;
@@ -523,10 +551,6 @@ __global__ __launch_bounds__(256) void k_meson_decay(DevTables T, PetiteDarkDesc
       O.meta[o] = TWO_BODY | 0x100;
     }
   }
-  double s = have ? w : 0.0;
-  for (int off = 32; off > 0; off >>= 1) s += shfl_f64(s, lane ^ off);
-  if (lane == 0 && s != 0.0) unsafeAtomicAdd(yield + 4, s);
-  wave_stat(stats, 1, have ? 1ull : 0ull);
 }
 
 // ------------------------------------------------------ m_V-continuous yield --

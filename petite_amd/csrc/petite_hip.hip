@@ -1741,74 +1741,88 @@ int petite_dark_reweight(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, uint
   HIP_TRY(hipMemsetAsync(ctr, 0, C_N * sizeof(unsigned long long), s));
   int64_t out_base = 0;
   const bool mesons = c->dark.meson_w[0] > 0.0 || c->dark.meson_w[1] > 0.0 || c->dark.meson_w[2] > 0.0;
-  double t_sel = 0.0, t_dr = 0.0, t_em = 0.0;
+  // the select / draw / emit pass serves the processes of SM particles (codes
+  // 8..11); with TwoBody_BSMDecay alone only the meson pass runs
+  bool sm_dark = false;
+  for (int a = 0; a < c->dark.n_active; ++a) sm_dark = sm_dark || c->dark.active[a] != dark::TWO_BODY;
+  double t_sel = 0.0, t_dr = 0.0, t_em = 0.0, t_me = 0.0;
   int64_t launches = 0;
   for (int64_t first = 0; first < n_sm; first += chunk) {
     const int64_t n = std::min(chunk, n_sm - first);
-    HIP_TRY(hipMemsetAsync(c->req_count, 0, PETITE_NPROC * sizeof(unsigned int), s));
-    HIP_TRY(hipMemsetAsync(ctr + C_DARK_N, 0, sizeof(unsigned long long), s));
-    HIP_TRY(hipMemsetAsync(ctr + C_REFILL, 0, PETITE_NPROC * sizeof(unsigned long long), s));  // draw refill
-    Requests R{c->req_slot, c->req_E, c->req_cap, c->req_count};
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], s));
-    hipLaunchKernelGGL(dark::k_dark_select, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c->dev, c->dark,
-                       (const double*)c->dark_blob, S, first, n, seed, c->ent, R, ctr + C_DARK_N, ctr + C_DARK_STATS);
-    HIP_TRY(hipGetLastError());
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], s));
-    unsigned int rq[PETITE_NPROC];
     unsigned long long n_e = 0;
-    HIP_TRY(hipMemcpyAsync(rq, c->req_count, sizeof(rq), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(&n_e, ctr + C_DARK_N, sizeof(n_e), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    if ((int64_t)n_e > c->ent.cap) return fail(PETITE_ERR_HIP, "petite_dark_reweight: entry scratch overflow");
-    stats->n_entries += (int64_t)n_e;
-    if (out_base + (int64_t)n_e > out->capacity) {
-      return fail(PETITE_ERR_CAPACITY, "petite_dark_reweight: output smaller than the entries (n_entries holds the count so far)");
-    }
-    DrawLaunch io{};
-    io.req_slot = c->req_slot;
-    io.req_E = c->req_E;
-    io.sid_src = c->ent.dsid;
-    io.req_cap = c->req_cap;
-    io.explicit_out = 1;
-    io.x_out = c->dres_x;
-    io.trials_out = c->dres_trials;
-    io.w_out = c->dres_w;
-    io.counters = ctr;
-    int64_t blocks = 0;
-    plan_draw(c, rq, io, blocks);
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[2], s));
-    rc = launch_draws(c, io, seed, s);
-    if (rc) return rc;
-    if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], s));
-    if (n_e > 0) {
-      dark::EmitOut O{(float4*)out->p, (float4*)out->r, out->parent, out->meta};
-      hipLaunchKernelGGL(dark::k_dark_emit, dim3((unsigned)((n_e + 255) / 256)), dim3(256), 0, s, c->dev, c->dark, S,
-                         c->ent, (int64_t)n_e, (const float*)c->dres_x, (const int32_t*)c->dres_trials, O, out_base,
-                         seed, c->dyield, ctr + C_DARK_STATS);
+    if (sm_dark) {
+      HIP_TRY(hipMemsetAsync(c->req_count, 0, PETITE_NPROC * sizeof(unsigned int), s));
+      HIP_TRY(hipMemsetAsync(ctr + C_DARK_N, 0, sizeof(unsigned long long), s));
+      HIP_TRY(hipMemsetAsync(ctr + C_REFILL, 0, PETITE_NPROC * sizeof(unsigned long long), s));  // draw refill
+      Requests R{c->req_slot, c->req_E, c->req_cap, c->req_count};
+      if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], s));
+      hipLaunchKernelGGL(dark::k_dark_select, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c->dev, c->dark,
+                         (const double*)c->dark_blob, S, first, n, seed, c->ent, R, ctr + C_DARK_N, ctr + C_DARK_STATS);
       HIP_TRY(hipGetLastError());
-    }
-    if (c->timing) {
-      HIP_TRY(hipEventRecord(c->ev[4], s));
-      HIP_TRY(hipEventSynchronize(c->ev[4]));
-      float ms = 0.0f;
-      HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-      t_sel += ms;
-      HIP_TRY(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-      t_dr += ms;
-      HIP_TRY(hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
-      t_em += ms;
+      if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], s));
+      unsigned int rq[PETITE_NPROC];
+      HIP_TRY(hipMemcpyAsync(rq, c->req_count, sizeof(rq), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(&n_e, ctr + C_DARK_N, sizeof(n_e), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      if ((int64_t)n_e > c->ent.cap) return fail(PETITE_ERR_HIP, "petite_dark_reweight: entry scratch overflow");
+      stats->n_entries += (int64_t)n_e;
+      if (out_base + (int64_t)n_e > out->capacity) {
+        return fail(PETITE_ERR_CAPACITY, "petite_dark_reweight: output smaller than the entries (n_entries holds the count so far)");
+      }
+      DrawLaunch io{};
+      io.req_slot = c->req_slot;
+      io.req_E = c->req_E;
+      io.sid_src = c->ent.dsid;
+      io.req_cap = c->req_cap;
+      io.explicit_out = 1;
+      io.x_out = c->dres_x;
+      io.trials_out = c->dres_trials;
+      io.w_out = c->dres_w;
+      io.counters = ctr;
+      int64_t blocks = 0;
+      plan_draw(c, rq, io, blocks);
+      if (c->timing) HIP_TRY(hipEventRecord(c->ev[2], s));
+      rc = launch_draws(c, io, seed, s);
+      if (rc) return rc;
+      if (c->timing) HIP_TRY(hipEventRecord(c->ev[3], s));
+      if (n_e > 0) {
+        dark::EmitOut O{(float4*)out->p, (float4*)out->r, out->parent, out->meta};
+        hipLaunchKernelGGL(dark::k_dark_emit, dim3((unsigned)((n_e + 255) / 256)), dim3(256), 0, s, c->dev, c->dark, S,
+                           c->ent, (int64_t)n_e, (const float*)c->dres_x, (const int32_t*)c->dres_trials, O, out_base,
+                           seed, c->dyield, ctr + C_DARK_STATS);
+        HIP_TRY(hipGetLastError());
+      }
+      if (c->timing) {
+        HIP_TRY(hipEventRecord(c->ev[4], s));
+        HIP_TRY(hipEventSynchronize(c->ev[4]));
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        t_sel += ms;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+        t_dr += ms;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev[3], c->ev[4]));
+        t_em += ms;
+      }
     }
     ++launches;
     out_base += (int64_t)n_e;
     if (mesons) {
       HIP_TRY(hipMemsetAsync(ctr + C_MESON_N, 0, sizeof(unsigned long long), s));
       dark::EmitOut O{(float4*)out->p, (float4*)out->r, out->parent, out->meta};
-      hipLaunchKernelGGL(dark::k_meson_decay, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c->dev, c->dark, S,
+      if (c->timing) HIP_TRY(hipEventRecord(c->ev[0], s));
+      hipLaunchKernelGGL(dark::k_meson_decay, dim3((unsigned)((n + dark::kMesonBlock - 1) / dark::kMesonBlock)),
+                         dim3(dark::kMesonBlock), 0, s, c->dev, c->dark, S,
                          first, n, seed, O, out_base, out->capacity, ctr + C_MESON_N, c->dyield, ctr + C_DARK_STATS);
       HIP_TRY(hipGetLastError());
+      if (c->timing) HIP_TRY(hipEventRecord(c->ev[1], s));
       unsigned long long n_m = 0;
       HIP_TRY(hipMemcpyAsync(&n_m, ctr + C_MESON_N, sizeof(n_m), hipMemcpyDeviceToHost, s));
       HIP_TRY(hipStreamSynchronize(s));
+      if (c->timing) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+        t_me += ms;
+      }
       stats->n_entries += (int64_t)n_m;
       if (out_base + (int64_t)n_m > out->capacity)
         return fail(PETITE_ERR_CAPACITY, "petite_dark_reweight: output smaller than the entries (n_entries holds the count so far)");
@@ -1827,6 +1841,7 @@ int petite_dark_reweight(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, uint
   stats->n_trials = (int64_t)hs[ST_TRIALS];
   stats->n_no_sample = (int64_t)hs[ST_NOSAMPLE];
   stats->ms_select = t_sel;
+  stats->ms_meson = t_me;
   stats->ms_draw = t_dr;
   stats->ms_emit = t_em;
   stats->launches = launches;
