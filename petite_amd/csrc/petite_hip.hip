@@ -887,6 +887,7 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
   int64_t oidx = 0;
   int2 info{0, 0};
   uint64_t sid = 0;
+  unsigned long long n_drop = 0;
   if (r < total) {
     int p = 0;
 #pragma unroll
@@ -937,12 +938,10 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
       }
       if (shower_energy && (p == COMP || p == ANN || p == MOLLER || p == BHABHA || p == MUONE))
         unsafeAtomicAdd(shower_energy + 4 * shower + 3, 1.0);
-      if (stats) {
-        const unsigned long long nd = (!keep[0]) + (!keep[1]);
-        if (nd) atomicAdd(stats + ST_DROPPED, nd);
-      }
+      n_drop = (!keep[0]) + (!keep[1]);
     }
   }
+  if (stats) wave_stat(stats, ST_DROPPED, n_drop);  // wave-uniform point: one atomic per wave
   const int nk = (int)keep[0] + (int)keep[1];
   // wave-aggregated slot allocation
   unsigned long long incl = nk;
