@@ -34,6 +34,19 @@ def transverse_position(particle, z):
     return [xf, yf]
 
 
+_GATHER_ROWS = 1 << 26
+
+
+def _take(t, idx):
+    """t.index_select(0, idx) in pieces whose output stays below 2^31 bytes:
+    on this ROCm/torch stack a single row gather with a larger output returned
+    wrong rows past that point."""
+    row = t[0:1].numel() * t.element_size()
+    if idx.numel() * row < (1 << 31):
+        return t.index_select(0, idx)
+    return torch.cat([t.index_select(0, idx[i:i + _GATHER_ROWS]) for i in range(0, idx.numel(), _GATHER_ROWS)])
+
+
 def _records(particle_list):
     """(p, r, kind, selection) on the device: float32 (n, 4) production
     momentum and (x, y, z, weight)."""
@@ -43,8 +56,8 @@ def _records(particle_list):
         return particle_list.p0.contiguous(), particle_list.r0.contiguous(), "batch", None
     if isinstance(particle_list, DarkBatch):
         idx = torch.as_tensor(particle_list.order(), device=particle_list.p.device, dtype=torch.long)
-        return (particle_list.p.index_select(0, idx).contiguous(),
-                particle_list.tensor("r").index_select(0, idx).contiguous(), "batch", idx)
+        return (_take(particle_list.p, idx).contiguous(), _take(particle_list.tensor("r"), idx).contiguous(),
+                "batch", idx)
     parts = list(particle_list)
     n = len(parts)
     p = np.zeros((n, 4), dtype=np.float32)
@@ -98,7 +111,7 @@ def detector_cut(particle_list, detector_positions, detector_radius, method="Sam
     keep = torch.nonzero(win).flatten()
     if keep.numel() == 0:
         return [[] for _ in range(n_det)] if method == "Sample" else [0.0 for _ in range(n_det)]
-    m = mask.index_select(0, keep)
+    m = _take(mask, keep)
     bits = torch.stack([((m >> d) & 1).bool() for d in range(n_det)])  # (n_det, n_filtered)
     if method == "Sample":
         if kind == "list":
