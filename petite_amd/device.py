@@ -150,6 +150,13 @@ class DeviceContext:
         prim = np.ascontiguousarray(primaries, dtype=PRIMARY_DTYPE)
         n = len(prim)
         dev = torch.device("cuda")
+        if n == 0:  # nothing to transport: an empty record (the C entry point rejects n = 0)
+            store = {k: torch.empty((0, 4) if k in ("p0", "r0", "pf", "rf") else (0,), device=dev,
+                                    dtype=torch.float32 if k in ("p0", "r0", "pf", "rf") else
+                                    (torch.int64 if k == "sid" else torch.int32))
+                     for k in ("p0", "r0", "pf", "rf", "pid", "parent", "meta", "shower", "sid")}
+            return store, 0, _lib.Stats().as_dict(), torch.zeros((0, 4), device=dev, dtype=torch.float64), \
+                _lib.PETITE_OK
         prim_t = torch.from_numpy(prim.view(np.uint8).copy()).to(dev)
         n_showers = int(prim["shower"].max()) + 1 if n else 0
         if capacity is None:

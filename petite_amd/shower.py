@@ -310,7 +310,8 @@ class Shower:
         if isinstance(primaries, np.ndarray) and primaries.dtype == PRIMARY_DTYPE:
             prim = primaries
         else:
-            prim = np.concatenate([self._primary_record(p, i, i) for i, p in enumerate(primaries)])
+            recs = [self._primary_record(p, i, i) for i, p in enumerate(primaries)]
+            prim = np.concatenate(recs) if recs else np.zeros(0, dtype=PRIMARY_DTYPE)
         keep = prim["E"] >= self.min_energy
         ctx = self.device_context(MS=GlobalMS)
         store, n, stats, energy, rc = ctx.run_showers(prim, self._seed if seed is None else seed, capacity)
