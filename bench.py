@@ -169,6 +169,7 @@ def main():
         if k + 1 < args.steps:
             del b
     torch.cuda.synchronize()
+    last_stats = b.stats
     dt = time.perf_counter() - t0
     K = args.steps
     steps_per_s = totals["steps"] / dt
@@ -239,6 +240,11 @@ def main():
         "particles_per_step": totals["particles"] / K,
         "particle_steps_per_step": totals["steps"] / K,
         "trials_per_draw": totals["trials"] / max(totals["draws"], 1),
+        "draws_by_process": {name: {"draws": d, "trials_per_draw": t / d}
+                             for name, d, t in zip(("Brem", "Ann", "PairProd", "Comp", "Moller", "Bhabha", "MuonE",
+                                                    "MuonBrem"), last_stats["n_draws_proc"],
+                                                   last_stats["n_trials_proc"])
+                             if d > 0},
         "kernels": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -112,6 +112,8 @@ typedef struct {
    * timing is enabled with petite_set_timing) */
   double ms_transport, ms_draw, ms_emit;
   int64_t launches_transport, launches_draw, launches_emit;
+  int64_t n_draws_proc[PETITE_NPROC];  /* draws per process code */
+  int64_t n_trials_proc[PETITE_NPROC]; /* candidates per process code */
 } PetiteStats;
 
 /* Create a context: copies the table blobs (host pointers) to the device. */

@@ -61,10 +61,13 @@ class Stats(C.Structure):
                 ("n_trials", C.c_int64), ("n_maxf_viol", C.c_int64), ("n_no_sample", C.c_int64),
                 ("n_iterations", C.c_int64), ("n_dropped", C.c_int64), ("ms_transport", C.c_double),
                 ("ms_draw", C.c_double), ("ms_emit", C.c_double), ("launches_transport", C.c_int64),
-                ("launches_draw", C.c_int64), ("launches_emit", C.c_int64)]
+                ("launches_draw", C.c_int64), ("launches_emit", C.c_int64),
+                ("n_draws_proc", C.c_int64 * 12), ("n_trials_proc", C.c_int64 * 12)]
 
     def as_dict(self):
-        return {f: (float(getattr(self, f)) if t is C.c_double else int(getattr(self, f))) for f, t in self._fields_}
+        return {f: (float(getattr(self, f)) if t is C.c_double else
+                    [int(v) for v in getattr(self, f)] if f.endswith("_proc") else int(getattr(self, f)))
+                for f, t in self._fields_}
 
 
 class DarkDesc(C.Structure):

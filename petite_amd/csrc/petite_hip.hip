@@ -1661,6 +1661,16 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   stats->n_no_sample = (int64_t)hs[ST_NOSAMPLE];
   stats->n_iterations = (int64_t)sweeps;
   stats->n_dropped = (int64_t)hs[ST_DROPPED];
+  {
+    unsigned long long pt[PETITE_NPROC], pd[PETITE_NPROC];
+    HIP_TRY(hipMemcpyAsync(pt, ctr + C_PTRIALS, sizeof(pt), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(pd, ctr + C_PDRAWS, sizeof(pd), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int p = 0; p < PETITE_NPROC; ++p) {
+      stats->n_trials_proc[p] = (int64_t)pt[p];
+      stats->n_draws_proc[p] = (int64_t)pd[p];
+    }
+  }
   stats->ms_transport = t_tr;
   stats->ms_draw = t_dr;
   stats->ms_emit = t_em;
