@@ -44,3 +44,10 @@ def test_load_refuses_other_files(tmp_path):
     np.savez(path, a=np.zeros(3))
     with pytest.raises(ValueError, match="not a petite_amd shower record"):
         ParticleBatch.load(path, device="cpu")
+
+
+def test_from_arrays_checks_shapes_before_touching_the_device():
+    with pytest.raises(ValueError):
+        ParticleBatch.from_arrays(np.zeros((3, 3)), 111)
+    with pytest.raises(ValueError):
+        ParticleBatch.from_arrays(np.zeros((3, 4)), 111, r=np.zeros((2, 3)))
