@@ -190,11 +190,11 @@ class Particle:
                 break
             try:
                 f_max, f = dalitz_information
-                accept = f(s12, s23, m1, m2, m3, M) > np.random.uniform(0, f_max)
-            except Exception:
+            except (TypeError, ValueError):
                 raise ValueError("Dalitz information must be a tuple of (max value, function) for "
-                                 "matrix-element-level sampling")
-            if accept:
+                                 "matrix-element-level sampling") from None
+            # errors raised by the matrix element itself propagate unchanged
+            if f(s12, s23, m1, m2, m3, M) > np.random.uniform(0, f_max):
                 break
         E1 = (M * M - s23 + m1 * m1) / (2.0 * M)
         E3 = (M * M - s12 + m3 * m3) / (2.0 * M)

@@ -145,6 +145,11 @@ def test_three_body_matrix_element_and_rest_isotropy():
     with pytest.raises(ValueError):
         rest.three_body_decay({"PID": 111}, {"PID": 111}, {"PID": 221}, dalitz_information="not flat")
 
+    def broken(*args):
+        raise ZeroDivisionError("inside the matrix element")
+    with pytest.raises(ZeroDivisionError):
+        rest.three_body_decay({"PID": 111}, {"PID": 111}, {"PID": 221}, dalitz_information=(1.0, broken))
+
 
 def test_decay_distance_law():
     np.random.seed(6)
