@@ -48,6 +48,8 @@ enum CounterIdx {
   C_DARK_N = 20,              // dark entries written in the current pass
   C_DARK_STATS = 21,          // [3] drate rows summing to zero, vectors written, bound-electron momentum failures
   C_MESON_N = 24,             // meson-decay entries written in the current pass
+  C_YIELD_OUT = 26,           // particles the current sweep's transport handed on to the next sweep
+  C_RPARITY = 27,             // which half of the resume buffer the current sweep reads
   C_STATS = 32,               // [ST_N] run statistics
   C_PTRIALS = 64,             // [PETITE_NPROC] candidates per process
   C_PDRAWS = 80,              // [PETITE_NPROC] draws per process
@@ -639,16 +641,42 @@ __device__ inline void charged_advance(const DevTables& T, double dist, double m
 // much as several substeps (global loads, atomics), and measured on MI355X
 // 44 of 64 lanes gives the shortest sweeps (32 and 56: 1.13x longer).
 constexpr int kBatchLanes = 44;
+// A particle that has taken kYieldSteps substeps in one launch hands its live
+// state on to the next sweep, which starts it first: the few long-lived
+// particles otherwise keep nearly empty waves running at the end of every
+// launch.  Its Philox blocks are indexed by its total substep count, so the
+// random numbers, and every result, are those of an uninterrupted transport.
+constexpr uint32_t kYieldSteps = 32;
+struct ResumeRec {
+  double E, dep, dz;
+  float nx, ny, nz, rx, ry, rz;
+  int32_t slot;
+  uint32_t k;
+};
+struct ResumeBuf {
+  ResumeRec* rec;  // [2][cap]: the half of the current sweep's parity is read, the other written
+  int64_t cap;
+};
 // range (non-null): the sweep's first slot and count, set on the device by
 // k_sweep_setup; blocks beyond what the count needs leave at once.
 __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int64_t first, int64_t n, uint64_t seed,
                                                    Requests R, ParentRecs PR, double* shower_energy,
                                                    unsigned long long* counters, const double* sig_lds_src,
-                                                   const int64_t* range) {
+                                                   const int64_t* range, ResumeBuf RB) {
+  // resumed particles (the previous sweep's hand-ons) come first in the queue
+  int64_t nres = 0;
+  const ResumeRec* rin = nullptr;
+  ResumeRec* rout = nullptr;
   if (range) {
     first = range[0];
     n = range[1];
-    if ((int64_t)blockIdx.x * 4 >= (n + 63) / 64) return;
+    if (RB.rec) {
+      nres = range[3];
+      const int64_t par = (int64_t)(counters[C_RPARITY] & 1ull);
+      rin = RB.rec + par * RB.cap;
+      rout = RB.rec + (1 - par) * RB.cap;
+    }
+    if ((int64_t)blockIdx.x * 4 >= (n + nres + 63) / 64) return;
   }
   HIP_DYNAMIC_SHARED(double, s_sig)
   for (int i = threadIdx.x; i < T.lds_doubles; i += blockDim.x) s_sig[i] = sig_lds_src[i];
@@ -665,6 +693,7 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
   double m = 0.0, pme = 0.0, E = 0.0, p = 0.0, dep = 0.0, dz = 0.0;
   float nx = 0.0f, ny = 0.0f, nz = 0.0f, rx = 0.0f, ry = 0.0f, rz = 0.0f, wgt = 0.0f;
   uint32_t k = 0;
+  uint32_t kl = 0;       // substeps taken in this launch
   bool closing = false;  // the next advance is the closing increment
   unsigned long long steps = 0;
   while (true) {
@@ -675,8 +704,36 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
     if (im && (__popcll(im) >= kBatchLanes || am == 0ull)) {
       const int64_t r = q.take(im, idle, counters + C_TRANSPORT);
       if (idle) {
-        if (r < n) {
-          slot = first + r;
+        if (r < nres) {
+          // a particle handed on by the previous sweep: constants from the
+          // store, live state from its resume record
+          const ResumeRec& R0 = rin[r];
+          slot = R0.slot;
+          pid = S.pid[slot];
+          sid = S.sid[slot];
+          shower = S.shower[slot];
+          gen = S.meta[slot] & 0xFFFF;
+          wgt = S.r0[slot].w;
+          ch = sum_channels(pid);
+          m = pid_mass(pid);
+          pme = fmax(fmax(min_calc(T, pid), T.d->min_energy), m);
+          E = R0.E;
+          dep = R0.dep;
+          dz = R0.dz;
+          nx = R0.nx;
+          ny = R0.ny;
+          nz = R0.nz;
+          rx = R0.rx;
+          ry = R0.ry;
+          rz = R0.rz;
+          k = R0.k;
+          p = sqrt(fmax(E * E - m * m, 0.0));
+          kl = 0;
+          closing = false;
+          act = true;
+        } else if (r < nres + n) {
+          slot = first + (r - nres);
+          kl = 0;
           const float4 p0 = S.p0[slot];
           const float4 r0 = S.r0[slot];
           pid = S.pid[slot];
@@ -846,6 +903,31 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
       } else if (E < pme) {
         closing = true;  // below the cutoff after this substep: close next
       }
+      // hand a long-lived particle on to the next sweep (space permitting)
+      const bool give = rout && !closing && act && ++kl >= kYieldSteps;
+      const unsigned long long gm = __ballot(give);
+      if (gm) {
+        const int leader = __ffsll((long long)gm) - 1;
+        unsigned long long gb = 0;
+        if (lane == leader) gb = atomicAdd(counters + C_YIELD_OUT, (unsigned long long)__popcll(gm));
+        gb = shfl_u64(gb, leader);
+        const int64_t gi = (int64_t)gb + __popcll(gm & ((1ull << lane) - 1ull));
+        if (give && gi < RB.cap) {
+          ResumeRec& W = rout[gi];
+          W.E = E;
+          W.dep = dep;
+          W.dz = dz;
+          W.nx = nx;
+          W.ny = ny;
+          W.nz = nz;
+          W.rx = rx;
+          W.ry = ry;
+          W.rz = rz;
+          W.slot = (int32_t)slot;
+          W.k = k;
+          act = false;  // the lane is free for the next record
+        }
+      }
     }
   }
   wave_stat(counters, C_STATS + ST_STEPS, steps);
@@ -984,7 +1066,7 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
 // the last sweep (at most `bound`, the rest waits for the next sweep), and the
 // per-sweep counters are cleared.  range = {first, n, end}.
 __global__ void k_sweep_setup(unsigned long long* ctr, int64_t* range, unsigned int* req_count, int64_t capacity,
-                              int64_t bound) {
+                              int64_t bound, int64_t resume_cap) {
   const int t = threadIdx.x;
   if (t < PETITE_NPROC) req_count[t] = 0u;
   if (t <= C_TRANSPORT) ctr[t] = 0ull;  // draw refill and transport refill counters
@@ -998,7 +1080,14 @@ __global__ void k_sweep_setup(unsigned long long* ctr, int64_t* range, unsigned 
     range[0] = first;
     range[1] = n;
     range[2] = first + n;
-    if (n > 0) ctr[C_SWEEPS] += 1ull;
+    // the previous transport's hand-ons; a reservation past the buffer kept
+    // its particle stepping, so only the first resume_cap records exist
+    int64_t nres = (int64_t)ctr[C_YIELD_OUT];
+    if (nres > resume_cap) nres = resume_cap;
+    ctr[C_YIELD_OUT] = 0ull;
+    ctr[C_RPARITY] ^= 1ull;
+    range[3] = nres;
+    if (n > 0 || nres > 0) ctr[C_SWEEPS] += 1ull;
   }
 }
 
@@ -1135,6 +1224,8 @@ struct PetiteCtx {
   hipEvent_t djoin[PETITE_NPROC] = {};
   // device-driven sweeps: {first, n, end} of the current sweep, and timing events
   int64_t* drange = nullptr;
+  ResumeRec* dresume = nullptr;  // [2][resume_cap]
+  int64_t resume_cap = 0;
   std::vector<hipEvent_t> tev;
   int n_cu = 256;
   double tbar[PETITE_NPROC];  // running mean of candidates per draw, per process
@@ -1422,6 +1513,7 @@ void petite_ctx_destroy(PetiteCtx* c) {
   for (hipEvent_t e : c->tev)
     if (e) (void)hipEventDestroy(e);
   if (c->drange) (void)hipFree(c->drange);
+  if (c->dresume) (void)hipFree(c->dresume);
   for (int p = 0; p < PETITE_NPROC; ++p) {
     if (c->djoin[p]) (void)hipEventDestroy(c->djoin[p]);
     if (c->dstream[p]) (void)hipStreamDestroy(c->dstream[p]);
@@ -1564,6 +1656,16 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   rc = ensure_parent_recs(c);
   if (rc) return rc;
   if (!c->drange) HIP_TRY(hipMalloc(&c->drange, 4 * sizeof(int64_t)));
+  {
+    const int64_t want = std::min<int64_t>(bound, (int64_t)1 << 20);
+    if (c->resume_cap < want) {
+      if (c->dresume) HIP_TRY(hipFree(c->dresume));
+      c->dresume = nullptr;
+      c->resume_cap = 0;
+      HIP_TRY(hipMalloc(&c->dresume, 2 * want * sizeof(ResumeRec)));
+      c->resume_cap = want;
+    }
+  }
   unsigned long long* ctr = c->counters;
   HIP_TRY(hipMemsetAsync(ctr, 0, C_N * sizeof(unsigned long long), s));
   HIP_TRY(hipMemsetAsync(c->drange, 0, 4 * sizeof(int64_t), s));
@@ -1611,14 +1713,16 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   double t_tr = 0.0, t_dr = 0.0, t_em = 0.0;
   int64_t queued = 0;
   unsigned long long h_store = n0;
-  int64_t h_range[3] = {0, 0, 0};
+  int64_t h_range[4] = {0, 0, 0, 0};
   while (true) {
     for (int g = 0; g < kSweepGroup; ++g) {
-      hipLaunchKernelGGL(k_sweep_setup, dim3(1), dim3(64), 0, s, ctr, c->drange, c->req_count, store->capacity, bound);
+      hipLaunchKernelGGL(k_sweep_setup, dim3(1), dim3(64), 0, s, ctr, c->drange, c->req_count, store->capacity, bound,
+                         c->resume_cap);
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 0], s));
       hipLaunchKernelGGL(k_transport, dim3(tr_blocks), dim3(256), lds, s, c->dev, S, (int64_t)0, (int64_t)0, seed, R,
-                         c->par, shower_energy, ctr, c->dsig_lds, (const int64_t*)c->drange);
+                         c->par, shower_energy, ctr, c->dsig_lds, (const int64_t*)c->drange,
+                         ResumeBuf{c->dresume, c->resume_cap});
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 1], s));
       rc = launch_draws_device(c, io, seed, s, muons);
@@ -1645,7 +1749,7 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
       }
     }
     if ((int64_t)h_store > store->capacity) break;  // children were dropped: report the size reached
-    if (h_range[1] == 0 && (int64_t)h_store == h_range[2]) break;  // the last sweep found nothing to do
+    if (h_range[1] == 0 && h_range[3] == 0 && (int64_t)h_store == h_range[2]) break;  // the last sweep found nothing to do
     if (queued > 1000000) return fail(PETITE_ERR_HIP, "petite_run_showers: sweep count runaway");
   }
   unsigned long long hs[ST_N], sweeps = 0;
