@@ -646,7 +646,7 @@ constexpr int kBatchLanes = 44;
 // particles otherwise keep nearly empty waves running at the end of every
 // launch.  Its Philox blocks are indexed by its total substep count, so the
 // random numbers, and every result, are those of an uninterrupted transport.
-constexpr uint32_t kYieldSteps = 32;
+constexpr uint32_t kYieldSteps = 40;
 struct ResumeRec {
   double E, dep, dz;
   float nx, ny, nz, rx, ry, rz;
