@@ -16,15 +16,25 @@ Philox4x32-10 streams, in the same order, as the HIP kernels
 (``oracle.philox``), which makes candidate-level comparisons possible.
 
 Parity pins (what checks each function; see tests/):
-* philox: Random123 known-answer vectors.
-* integrands + map transform: sigma-table integral identity against the
-  shipped ``sm_xsec.pkl``/``dark_xsec.pkl`` values; closed-form total cross
-  sections (Moller, Bhabha, mu-e) and Klein-Nishina.
-* draw (accept/reject): brute-force histogram of accepted samples against the
-  normalised integrand on a fine grid.
-* kinematics: energy conservation / on-shell invariants per vertex.
-* MCS: Rayleigh law of the space angle (KS), |p| preserved.
+* philox: Random123 known-answer vectors (test_oracle_philox.py).
+* integrands + map transform: the sigma-table integral identity against the
+  shipped ``sm_xsec.pkl``/``dark_xsec.pkl`` values (test_oracle_tables.py);
+  closed-form total cross sections (Moller, Bhabha, mu-e) and Klein-Nishina
+  (test_oracle_closed_forms.py).
+* draw (accept/reject): the accepted points' law -- 1-D processes against f
+  normalised on a fine grid, Brem/PairProd marginals against the VEGAS
+  J f-weighted estimate, at max_F above sup J f (test_oracle_draw_law.py).
+* kinematics: vertex invariants (test_oracle_physics.py), and the invariants
+  recomputed from the four-vectors (momentum transfer q^2, Mandelstam t,
+  the annihilation cut, the dark vector's angle) against the variables the
+  integrands assign to the same map point (test_oracle_kinematics_pins.py).
+* MCS: the Lynch-Dahl width against Highland's formula
+  (test_oracle_moliere_pin.py), the Rayleigh law of the space angle and |p|
+  preservation (test_oracle_physics.py).
+* DarkAnn: its integral against the reference's luminosity table
+  lumi_integral_list (test_oracle_dark_ann_lumi.py).
 * transport: exponential free-path law (KS), per-shower energy budget.
-* dark weights/drate: stored ``dark_weights.pkl``/``dark_drate.pkl`` caches.
+* dark weights/drate: the stored ``dark_weights.pkl``/``dark_drate.pkl``
+  caches and quadrature (test_dark_tables.py, test_oracle_dark_yield.py).
 Functions without such a check say "parity unpinned" in their docstring.
 """

@@ -30,8 +30,11 @@ def g2_elastic(Z, t):
     return Z ** 2 * a0 ** 4 * t ** 2 / (1 + a0 ** 2 * t) ** 2
 
 
-def brem(x, ev):
-    """Brem / MuonBrem, all_processes.py:139-206."""
+def brem_vars(x, ev):
+    """The Brem / MuonBrem integrand's kinematic variables of the map point x
+    (all_processes.py:139-172): photon energy w, the scaled angles d, d' of
+    photon and lepton, the azimuth ph between them, the final lepton energy
+    epp and the momentum transfer q^2 to the nucleus."""
     ep = ev["E_inc"]
     egmin = ev["Eg_min"]
     m = ev["m_lepton"]
@@ -41,10 +44,20 @@ def brem(x, ev):
     dp = ep / (2 * m) * (x2 - x3)
     ph = (x4 - 0.5) * 2 * PI
     epp = ep - w
-    ok = (egmin < w) & (w < ep - m) & (m < epp) & (epp < ep) & (d > 0.0) & (dp > 0.0)
     with np.errstate(all="ignore"):
         qsq = m ** 2 * ((d ** 2 + dp ** 2 - 2 * d * dp * np.cos(ph))
                         + m ** 2 * ((1 + d ** 2) / (2 * ep) - (1 + dp ** 2) / (2 * epp)) ** 2)
+    return w, d, dp, ph, epp, qsq
+
+
+def brem(x, ev):
+    """Brem / MuonBrem, all_processes.py:139-206."""
+    ep = ev["E_inc"]
+    egmin = ev["Eg_min"]
+    m = ev["m_lepton"]
+    w, d, dp, ph, epp, qsq = brem_vars(x, ev)
+    ok = (egmin < w) & (w < ep - m) & (m < epp) & (epp < ep) & (d > 0.0) & (dp > 0.0)
+    with np.errstate(all="ignore"):
         PF = 8.0 / PI * alpha_em * (alpha_em / m) ** 2 * (epp * m ** 4) / (w * ep * qsq ** 2) * d * dp
         jac = PI * ep ** 2 * (ep - m - egmin) / m ** 2
         FF = g2_elastic(ev["Z_T"], qsq)
@@ -56,8 +69,11 @@ def brem(x, ev):
     return np.where(ok, out, 0.0)
 
 
-def pairprod(x, ev):
-    """PairProd, all_processes.py:531-622."""
+def pairprod_vars(x, ev):
+    """The PairProd integrand's kinematic variables of the map point x
+    (all_processes.py:531-580): positron and electron energies epp, epm, their
+    scaled angles dp, dm, the azimuth ph between them and the momentum
+    transfer q^2 / m_e^2 (qq) to the nucleus."""
     w = ev["E_inc"]
     m = m_electron
     x1, x2, x3, x4 = x.T
@@ -66,10 +82,19 @@ def pairprod(x, ev):
     dm = w / (2 * m) * (x2 - x3)
     ph = x4 * 2 * PI
     epm = w - epp
-    ok = (m < epm) & (epm < w) & (m < epp) & (epp < w) & (dm > 0.0) & (dp > 0.0)
     with np.errstate(all="ignore"):
         qq = (dp ** 2 + dm ** 2 + 2.0 * dp * dm * np.cos(ph)) + m ** 2 * (
             (1.0 + dp ** 2) / (2.0 * epp) + (1.0 + dm ** 2) / (2.0 * epm)) ** 2
+    return epp, epm, dp, dm, ph, qq
+
+
+def pairprod(x, ev):
+    """PairProd, all_processes.py:531-622."""
+    w = ev["E_inc"]
+    m = m_electron
+    epp, epm, dp, dm, ph, qq = pairprod_vars(x, ev)
+    ok = (m < epm) & (epm < w) & (m < epp) & (epp < w) & (dm > 0.0) & (dp > 0.0)
+    with np.errstate(all="ignore"):
         PF = 8.0 / PI * alpha_em * (alpha_em / m) ** 2 * epp * epm / (w ** 3 * qq ** 2) * dp * dm
         jac = PI * w ** 2 * (w - 2 * m) / m ** 2
         FF = g2_elastic(ev["Z_T"], m ** 2 * qq)
@@ -81,6 +106,18 @@ def pairprod(x, ev):
     return np.where(ok, out, 0.0)
 
 
+def compton_t(x, ev):
+    """The Mandelstam t the Compton / DarkComp integrand assigns to cos(theta)
+    = x[:, 0] (all_processes.py:660-668)."""
+    Eg = ev["E_inc"]
+    mV = ev.get("mV", 0.0)
+    m = m_electron
+    ct = x[:, 0]
+    s = m ** 2 + 2 * Eg * m
+    R = np.sqrt(m ** 4 + (mV ** 2 - s) ** 2 - 2 * m ** 2 * (mV ** 2 + s))
+    return -0.5 * (m ** 4 + s * (-(mV ** 2) + s + ct * R) - m ** 2 * (mV ** 2 + 2 * s + ct * R)) / s
+
+
 def compton(x, ev):
     """Comp / DarkComp (free electron at rest), all_processes.py:624-742."""
     Eg = ev["E_inc"]
@@ -90,9 +127,8 @@ def compton(x, ev):
     s = m ** 2 + 2 * Eg * m
     if s < (m + mV) ** 2:
         return np.zeros(len(ct))
-    R = np.sqrt(m ** 4 + (mV ** 2 - s) ** 2 - 2 * m ** 2 * (mV ** 2 + s))
     jac = (s - m ** 2) / (2 * s) * np.sqrt((s - mV ** 2) ** 2 - 2 * m ** 2 * (s + mV ** 2) + m ** 4)
-    t = -0.5 * (m ** 4 + s * (-(mV ** 2) + s + ct * R) - m ** 2 * (mV ** 2 + 2 * s + ct * R)) / s
+    t = compton_t(x, ev)
     PF = 2.0 * PI * alpha_em ** 2 / (s - m ** 2) ** 2
     if mV == 0.0:
         T1 = (6.0 * m ** 2 * s + 3.0 * m ** 4 - s ** 2) / ((m ** 2 - s) * (-(m ** 2) + s + t))
@@ -158,14 +194,22 @@ def bhabha(x, ev):
     return np.where(ok, val, 0.0)
 
 
+def muone_t(x, ev):
+    """The Mandelstam t (electron side) the mu e integrand assigns to
+    cos(theta) = x[:, 0] (all_processes.py:858-862)."""
+    Emu = ev["E_inc"]
+    s = m_electron ** 2 + m_muon ** 2 + 2 * m_electron * Emu
+    ct = x[:, 0]
+    return -2.0 * (1 - ct) * ((s + m_electron ** 2 - m_muon ** 2) ** 2 / (4.0 * s) - m_electron ** 2)
+
+
 def muone(x, ev):
     """mu e -> mu e on an electron at rest, all_processes.py:842-886."""
     Emu = ev["E_inc"]
     s = m_electron ** 2 + m_muon ** 2 + 2 * m_electron * Emu
     DE = ev.get("Ee_min", 0.010)
     t_limit = 2.0 * m_electron * (m_electron - DE)
-    ct = x[:, 0]
-    t = -2.0 * (1 - ct) * ((s + m_electron ** 2 - m_muon ** 2) ** 2 / (4.0 * s) - m_electron ** 2)
+    t = muone_t(x, ev)
     ok = t < t_limit
     with np.errstate(all="ignore"):
         val = (16 * PI ** 2 * alpha_em ** 2 * (s ** 2 + 2 * (m_electron ** 2 + m_muon ** 2) * (2 * t + m_muon ** 2 - 3 * m_electron ** 2)
@@ -187,6 +231,16 @@ def G_el_inel_over_t2(ev, t):
     return Z ** 2 * c1 ** 2 * (Gel + Ginel)
 
 
+def dark_brem_vars(x, ev):
+    """The DarkBrem / DarkMuonBrem integrand's vector kinematics of the map
+    point x (all_processes.py:225-236): 1 - cos(theta_V), cos(theta_V) and
+    |k_V| for the energy fraction x[:, 0] and log10(1 - cos theta_V) = x[:, 1]."""
+    xx, l1mct = x[:, 0], x[:, 1]
+    one_minus_costheta = 10 ** l1mct
+    k = np.sqrt(np.fabs((xx * ev["E_inc"]) ** 2 - ev["mV"] ** 2))
+    return one_minus_costheta, 1.0 - one_minus_costheta, k
+
+
 def dark_brem(x, ev):
     """DarkBrem / DarkMuonBrem exact tree level, 'Log' method, all_processes.py:208-372."""
     m_lepton = ev["m_lepton"]
@@ -195,11 +249,9 @@ def dark_brem(x, ev):
     MT = ev["mT"]
     xx, l1mct, lttilde = x.T
     with np.errstate(all="ignore"):
-        one_minus_costheta = 10 ** l1mct
-        costheta = 1.0 - one_minus_costheta
+        one_minus_costheta, costheta, k = dark_brem_vars(x, ev)
         ttilde = 10 ** lttilde
         Jacobian = one_minus_costheta * ttilde * np.log(10.0) ** 2
-        k = np.sqrt(np.fabs((xx * Ebeam) ** 2 - mV ** 2))
         p = np.sqrt(Ebeam ** 2 - m_lepton ** 2)
         V = np.sqrt(p ** 2 + k ** 2 - 2 * p * k * costheta)
         utilde = -2 * (xx * Ebeam ** 2 - k * p * costheta) + mV ** 2

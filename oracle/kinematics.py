@@ -6,7 +6,9 @@ np.random.uniform(0, 2 pi) inside each function (here 2 pi * u).  Each
 function returns (first, second) four-vectors in the reference's order.
 
 Pinned by: energy conservation and on-shell invariants per vertex
-(tests/test_oracle_kinematics.py).
+(tests/test_oracle_physics.py), and the momentum transfer, Mandelstam t,
+annihilation cut and dark-vector angle recomputed from the four-vectors
+against the integrands' variables (tests/test_oracle_kinematics_pins.py).
 """
 import math
 

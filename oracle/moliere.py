@@ -8,8 +8,9 @@ variate; it is drawn here as s * sqrt(-2 ln u_r), with the azimuth 2 pi u_a
 and the sign s = +1 when bit 0 of the azimuth word is set (words of the
 substep's or the closing increment's Philox block, see oracle.shower).
 
-Pinned by: Rayleigh law of the space angle (KS) and |p| preservation
-(tests/test_oracle_physics.py).
+Pinned by: the width theta0 against Highland's formula
+(tests/test_oracle_moliere_pin.py), the Rayleigh law of the space angle (KS)
+and |p| preservation (tests/test_oracle_physics.py).
 """
 import math
 

@@ -20,7 +20,7 @@ Counting convention: one particle-step is one charged substep (shower.py
 increment (583-598) or one photon flight (538-553).
 
 Pinned by: exponential free-path law (KS), per-shower energy budget, and the
-reference's multiplicity anchor (tests/test_oracle_shower.py).
+reference's multiplicity anchor (tests/test_oracle_physics.py).
 """
 import math
 

@@ -16,8 +16,9 @@ LU_Key = argmin|E_i - E| + 1.  Uniforms come from the Philox streams of
 ``oracle.philox`` (PURPOSE_VEGAS_A, one block per candidate).
 
 Parity of the map: "parity unpinned" at the bit level for vegas itself (not
-installed); pinned through the sigma-table integral identity and the
-accept/reject law tests.
+installed); pinned through the sigma-table integral identity
+(tests/test_oracle_tables.py) and the law of the accepted points
+(tests/test_oracle_draw_law.py).
 """
 import numpy as np
 
