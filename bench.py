@@ -53,25 +53,8 @@ def measured_traffic(kernel, alg_bytes_per_launch):
 
 
 def make_primaries(n=N_PRIMARIES, seed=SEED):
-    from petite_amd.batch import PRIMARY_DTYPE
-    from petite_amd.constants import m_electron
-    rng = np.random.default_rng(seed)
-    pid = rng.choice(np.array([11, -11, 22], dtype=np.int32), size=n)
-    E = np.exp(rng.uniform(np.log(0.1), np.log(100.0), n))
-    tx = rng.normal(0.0, 1e-3, n)
-    ty = rng.normal(0.0, 1e-3, n)
-    m = np.where(pid == 22, 0.0, m_electron)
-    p = np.sqrt(E * E - m * m)
-    nrm = np.sqrt(tx * tx + ty * ty + 1.0)
-    prim = np.zeros(n, dtype=PRIMARY_DTYPE)
-    prim["E"] = E
-    prim["px"], prim["py"], prim["pz"] = p * tx / nrm, p * ty / nrm, p / nrm
-    prim["mass"] = m
-    prim["weight"] = 1.0
-    prim["pid"] = pid
-    prim["shower"] = np.arange(n, dtype=np.int32)
-    prim["sid"] = np.arange(n, dtype=np.uint64)
-    return prim
+    from petite_amd.workloads import batch_primaries
+    return batch_primaries(n, seed)
 
 
 def cpu_baseline(prim, maxf, budget_s=20.0):

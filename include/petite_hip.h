@@ -24,6 +24,7 @@ extern "C" {
 #define PETITE_ERR_HIP 2
 #define PETITE_ERR_NO_SAMPLE 3 /* draw_sample's "No Sample Found" (src/PETITE/shower.py:460-461) */
 #define PETITE_ERR_CAPACITY 4  /* particle store too small for the shower output */
+#define PETITE_ERR_INTERNAL 5  /* a device queue overflowed its region (a library bug, never the caller's input) */
 
 #define PETITE_NPROC 12 /* Brem Ann PairProd Comp Moller Bhabha MuonE MuonBrem DarkBrem DarkAnn DarkComp DarkMuonBrem */
 
@@ -125,6 +126,34 @@ const char* petite_last_error(void);
  * No counterpart in the reference: it feeds the roofline figures. */
 int petite_set_timing(PetiteCtx* ctx, int32_t on);
 
+/* Launch and queue sizes of petite_run_showers.  No counterpart in the
+ * reference, whose driver is a sequential Python FIFO (src/PETITE/shower.py:
+ * 634-706); the values change speed only, never results: every particle draws
+ * from its own Philox stream.  A field <= 0 selects the built-in default
+ * (given in brackets).  Block counts are in waves per compute unit (4 waves
+ * per block). */
+typedef struct {
+  int32_t batch_lanes;       /* transport: lanes of a wave that must wait for a refill / finish
+                              * before the wave runs one (1..64) [44] */
+  int32_t yield_steps;       /* transport: substeps of one particle in one launch before it is handed
+                              * on to the next sweep [40]; a value above 2^30 never hands on */
+  int32_t transport_waves;   /* transport waves per CU [20] */
+  int32_t emit_waves;        /* emission waves per CU [32] */
+  int32_t draw_waves[8];     /* draw waves per CU for SM process code p [Brem 16, PairProd 8, others 2] */
+  int32_t min_group_lanes;   /* draws: the fewest lanes that share one request (1, 2, 4, .., 64) [4] */
+  int32_t sweep_group;       /* sweeps queued per host check (1..64) [16] */
+  int32_t group_requests;    /* draws with fewer requests than lanes: requests a lane group is sized to
+                              * serve in one launch (1..16) [4] */
+  int32_t pad;
+  int64_t sweep_bound;      /* particles per sweep, resumed ones included (>= 64) [2^23] */
+  int64_t resume_cap;        /* hand-on records per sweep (1..sweep_bound) [min(sweep_bound, 2^20)] */
+} PetiteTuning;
+
+/* Set (tuning != NULL) or reset to the defaults (NULL) the context's launch
+ * sizes; petite_get_tuning reports the values in effect (defaults filled in). */
+int petite_set_tuning(PetiteCtx* ctx, const PetiteTuning* tuning);
+int petite_get_tuning(const PetiteCtx* ctx, PetiteTuning* tuning);
+
 /* max_F table of one process for the context material: the largest J*f over
  * n_samples map candidates at each grid energy (utilities/find_maxes.py:55-119).
  * Writes the device copy used by the draws; maxf_host/mean_host (n_E doubles,
@@ -163,6 +192,47 @@ int petite_eval_points(PetiteCtx* ctx, const int32_t* proc, const double* E, con
  * number of atomic-electron targets (in units of m_e). */
 int petite_run_showers(PetiteCtx* ctx, const PetitePrimary* prim, int64_t n, uint64_t seed, PetiteStore* store,
                        double* shower_energy, PetiteStats* stats, PetiteStream stream);
+
+/* Single steps of the shower, for the reference's lower-level methods.
+ *
+ * One particle at a hard interaction: its final four-momentum (pf), position
+ * (rf), weight, species, generation and Philox stream id. */
+typedef struct {
+  double E, px, py, pz;
+  double x, y, z;
+  double weight;
+  int32_t pid;
+  int32_t generation;
+  uint64_t sid;
+} PetiteVertex;
+
+/* Shower.sample_scattering (src/PETITE/shower.py:467-507) for n vertices
+ * (device arrays): vertex i interacts by SM process proc[i] (codes 0..7; the
+ * caller has checked the threshold of shower.py:469-470): draw_sample at
+ * E = v[i].E from stream v[i].sid, the final state rotated onto (px, py, pz),
+ * both children written to out entries 2i and 2i+1 (p0 = pf, r0 = rf =
+ * the vertex position with its weight, parent = i, meta as in
+ * petite_run_showers, stream ids derived from v[i].sid) whatever their
+ * energy.  trials_out[i] (device) receives the candidate count (<= 0: no
+ * sample; that vertex's entries are zero); PETITE_ERR_NO_SAMPLE when any
+ * draw found none. */
+int petite_sample_scattering(PetiteCtx* ctx, const PetiteVertex* v, const int32_t* proc, int64_t n, uint64_t seed,
+                             PetiteStore* out, int32_t* trials_out, PetiteStream stream);
+
+/* Shower.propagate_particle (src/PETITE/shower.py:509-601) as generate_shower
+ * calls it (606-660): photons one exponential flight without scattering,
+ * charged leptons substeps with the material's dE/dx and, if the context was
+ * created with MS, multiple scattering.  The first n records of `recs` (p0,
+ * r0 with the weight, pid, meta generation, shower, sid: a store laid out as
+ * petite_run_showers writes it) receive pf and rf (rf.w = the particle-steps
+ * taken); nothing else is written. */
+int petite_propagate(PetiteCtx* ctx, PetiteStore* recs, int64_t n, uint64_t seed, PetiteStream stream);
+
+/* DarkShower.draw_pe_sample (src/PETITE/dark_shower.py:706-719): the bound
+ * electron's momentum [GeV] from its 1s density (effective charge Zeff), one
+ * per stream id (device arrays of n); -1 where 65536 accept/reject pairs
+ * found none.  No context: the sampler needs no tables. */
+int petite_draw_pe(const uint64_t* sid, int64_t n, uint64_t seed, double Zeff, double* pe_out, PetiteStream stream);
 
 /* ----------------------------------------------------------- dark sector --
  * Weight families (GetBSMWeights, src/PETITE/dark_shower.py:595-647):
@@ -247,9 +317,10 @@ int petite_dark_reweight(PetiteCtx* ctx, const PetiteStore* sm, int64_t n_sm, ui
  * W_P = GetBSMWeights (src/PETITE/dark_shower.py:595-647) with eps^2 = 1.  For
  * m_V between trained masses m_j < m_{j+1}, with t = ln(m_V/m_j)/ln(m_{j+1}/m_j),
  * each particle's weight is W_j^(1-t) W_{j+1}^t when both are positive and
- * (1-t) W_j + t W_{j+1} otherwise; t is clamped to [0, 1] outside the trained
- * range (there dY/dm_V = 0). */
-#define PETITE_MAX_MASSES 8
+ * (1-t) W_j + t W_{j+1} otherwise.  Masses outside [lightest, heaviest
+ * trained mass] are rejected (PETITE_ERR_ARG): there is nothing to
+ * interpolate. */
+#define PETITE_MAX_MASSES 16
 
 /* Dark tables of n_mass trained masses (ascending), one descriptor each; all
  * offsets index the one float64 blob (host, n doubles).  descs[j].eps2 should

@@ -30,7 +30,7 @@ from .integrands import INTEGRAND
 from .kinematics import compton_bound, l_to_lV, radiative_return
 from .moliere import scatter
 from .shower import lose_energy, rotation_matrix
-from .tables import Grid, interp0, load_pack
+from .tables import Grid, dark_packs, grid_from, interp0, load_pack
 
 DARK_CODE = {"DarkBrem": 8, "DarkAnn": 9, "DarkComp": 10, "DarkMuonBrem": 11}
 
@@ -171,11 +171,13 @@ class LogInterp:
 
 class OracleDark:
     def __init__(self, sm_tables, mV_in_GeV, mode="exact", Zeff=29.508, kinetic_mixing=1.0, table_set="100GeV",
-                 below="clamp", use_cache=True, with_integrals=True):
-        z, meta = load_pack("dark", table_set)
+                 below="clamp", use_cache=True, with_integrals=True, grids=None, masses="shipped"):
+        """grids: {process: (E, nodes, node counts)} to use instead of the
+        packs' maps (tests pass the product's maps, so both sides draw from
+        the same map)."""
+        z, self.mV_list = dark_packs(table_set, masses)
         self.T = sm_tables
         T = sm_tables
-        self.mV_list = meta["masses"]
         if mode == "exact":
             self.mV = closest_lesser_value(self.mV_list, mV_in_GeV, below)
             self.mV_est = self.mV
@@ -233,6 +235,9 @@ class OracleDark:
         self._grids = {}
         for P in DARK_CODE:
             gk = f"{P}__{key}"
+            if grids is not None and P in grids:
+                self._grids[P] = grid_from(*grids[P])
+                continue
             src = z
             if f"E__{gk}" not in z.files and table_set != "100GeV":
                 src, _ = load_pack("dark", "100GeV")  # maps missing from the 400 GeV set

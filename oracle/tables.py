@@ -26,6 +26,46 @@ def load_pack(kind, table_set):
     return z, meta
 
 
+class MergedPack:
+    """npz packs read as one: the union of their files, each item from the
+    first pack that holds it."""
+
+    def __init__(self, packs):
+        self.packs = list(packs)
+        self.files = [f for z in self.packs for f in z.files]
+
+    def __getitem__(self, key):
+        for z in self.packs:
+            if key in z.files:
+                return z[key]
+        raise KeyError(key)
+
+
+def dark_packs(table_set, masses="shipped"):
+    """(pack, masses) of the dark tables: the shipped pack, and with
+    masses="all" also the cross sections of the generated mass pack
+    (dark_<set>_masses.npz, made on the GPU by the product's tablegen tool)."""
+    z, meta = load_pack("dark", table_set)
+    mv = [float(m) for m in meta["masses"]]
+    path = os.path.join(DATA_DIR, f"dark_{table_set}_masses.npz")
+    if masses == "all" and os.path.exists(path):
+        g = np.load(path)
+        gm = json.loads(bytes(g["meta_json"]).decode())["masses"]
+        return MergedPack([z, g]), sorted(set(mv) | {float(m) for m in gm})
+    return z, mv
+
+
+def grid_cuts(meta):
+    """(Eg_min, Ee_min) of a table set: the Brem grid's, else the first grid's
+    that records them."""
+    if meta.get("Eg_min") is not None and meta.get("Ee_min") is not None:
+        return float(meta["Eg_min"]), float(meta["Ee_min"])
+    for g in meta["grids"].values():
+        if g and g.get("Eg_min") is not None and g.get("Ee_min") is not None:
+            return float(g["Eg_min"]), float(g["Ee_min"])
+    raise ValueError("no grid of the table set records Eg_min / Ee_min")
+
+
 def interp0(xg, yg, x):
     """scipy.interpolate.interp1d(xg, yg, fill_value=0, bounds_error=False)(x)."""
     x = np.asarray(x, dtype=np.float64)
@@ -53,10 +93,24 @@ class Grid:
         return min(max(k, 0), len(self.E) - 1)
 
 
+def grid_from(E, nodes, cnt):
+    """A Grid from (energies, nodes of every energy back to back, node counts)."""
+    cnt = np.asarray(cnt)
+    stride = int(cnt[0].sum())
+    offs = np.zeros_like(cnt)
+    offs[:, 0] = np.arange(len(E)) * stride
+    for d in range(1, cnt.shape[1]):
+        offs[:, d] = offs[:, d - 1] + cnt[:, d - 1]
+    return Grid(E, np.asarray(nodes, dtype=np.float64), offs, cnt)
+
+
 class SMTables:
     """Per-material SM tables: n*sigma interpolators, thresholds, grids."""
 
-    def __init__(self, material, table_set="100GeV"):
+    def __init__(self, material, table_set="100GeV", grids=None):
+        """grids: {process: (E, nodes, node counts)} to use instead of the
+        packs' maps (a map sets only the draws' efficiency; tests pass the
+        product's maps so that both sides draw from the same one)."""
         z, meta = load_pack("sm", table_set)
         self.meta = meta
         self.material = material
@@ -66,8 +120,10 @@ class SMTables:
         self.dEdx = self.dEdx_MeVcm * 0.1  # GeV/m (shower.py:649)
         self.nZ = self.rho / m_proton_grams / self.A  # shower.py:202-208
         self.ne = self.nZ * self.Z
-        self.Eg_min = meta["Eg_min"]
-        self.Ee_min = meta["Ee_min"]
+        # the cuts come from the Brem grid's parameters (shower.py:214-215); the
+        # 400 GeV set ships no Brem grid, so they come from another grid of
+        # the set (all carry the same generator parameters)
+        self.Eg_min, self.Ee_min = grid_cuts(meta)
         xs = {P: np.array(z[f"xsec__{P}__{material}"]) for P in SM_PROCESSES}
         self.xsec_raw = xs
         mue = xs["MuonE"]
@@ -99,17 +155,19 @@ class SMTables:
         ns["MuonE"] = (me_E, self.ne * GeVsqcm2 * I.sigma_muone(me_E, self.Ee_min))
         ns["MuonBrem"] = (xs["MuonBrem"][:, 0], self.nZ * GeVsqcm2 * xs["MuonBrem"][:, 1])
         self.nsigma_tab = ns
-        # grids
+        # grids: given ones, else the set's, else (maps the set lacks) the 100 GeV set's
         self.grids = {}
         fallback = None
         for P in SM_PROCESSES:
-            if f"E__{P}" in z.files:
-                self.grids[P] = Grid(z[f"E__{P}"], z[f"nodes__{P}"], z[f"nodeoff__{P}"], z[f"nodecnt__{P}"])
-            else:
+            if grids is not None and P in grids:
+                self.grids[P] = grid_from(*grids[P])
+                continue
+            src = z
+            if f"E__{P}" not in z.files:
                 if fallback is None:
                     fallback, _ = load_pack("sm", "100GeV")
-                self.grids[P] = Grid(fallback[f"E__{P}"], fallback[f"nodes__{P}"],
-                                     fallback[f"nodeoff__{P}"], fallback[f"nodecnt__{P}"])
+                src = fallback
+            self.grids[P] = Grid(src[f"E__{P}"], src[f"nodes__{P}"], src[f"nodeoff__{P}"], src[f"nodecnt__{P}"])
 
     def nsigma(self, process, E):
         xg, yg = self.nsigma_tab[process]

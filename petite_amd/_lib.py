@@ -16,12 +16,14 @@ PETITE_ERR_ARG = 1
 PETITE_ERR_HIP = 2
 PETITE_ERR_NO_SAMPLE = 3
 PETITE_ERR_CAPACITY = 4
+PETITE_ERR_INTERNAL = 5
 
 EXPORTED = ["petite_ctx_create", "petite_ctx_destroy", "petite_last_error", "petite_maxf_scan", "petite_set_maxf",
             "petite_vegas_draw", "petite_eval_points", "petite_run_showers", "petite_set_timing", "petite_dark_setup",
             "petite_dark_reweight", "petite_yield_setup", "petite_dark_yield", "petite_detector_cut",
-            "petite_map_accumulate", "petite_set_grid"]
-MAX_MASSES = 8
+            "petite_map_accumulate", "petite_set_grid", "petite_set_tuning", "petite_get_tuning", "petite_sample_scattering",
+            "petite_propagate", "petite_draw_pe"]
+MAX_MASSES = 16
 DARK_FAMILIES = ["brem_elec", "brem_positron", "muon_brem", "annihilation"]
 DARK_DRATE_BINS = 10
 
@@ -50,6 +52,12 @@ class Primary(C.Structure):
                 ("weight", C.c_double), ("pid", C.c_int32), ("shower", C.c_int32), ("sid", C.c_uint64)]
 
 
+class Vertex(C.Structure):
+    _fields_ = [("E", C.c_double), ("px", C.c_double), ("py", C.c_double), ("pz", C.c_double),
+                ("x", C.c_double), ("y", C.c_double), ("z", C.c_double), ("weight", C.c_double),
+                ("pid", C.c_int32), ("generation", C.c_int32), ("sid", C.c_uint64)]
+
+
 class Store(C.Structure):
     _fields_ = [("p0", C.c_void_p), ("r0", C.c_void_p), ("pf", C.c_void_p), ("rf", C.c_void_p),
                 ("pid", C.c_void_p), ("parent", C.c_void_p), ("meta", C.c_void_p), ("shower", C.c_void_p),
@@ -68,6 +76,30 @@ class Stats(C.Structure):
         return {f: (float(getattr(self, f)) if t is C.c_double else
                     [int(v) for v in getattr(self, f)] if f.endswith("_proc") else int(getattr(self, f)))
                 for f, t in self._fields_}
+
+
+class Tuning(C.Structure):
+    """PetiteTuning: launch and queue sizes of petite_run_showers (0 = default)."""
+    _fields_ = [("batch_lanes", C.c_int32), ("yield_steps", C.c_int32), ("transport_waves", C.c_int32),
+                ("emit_waves", C.c_int32), ("draw_waves", C.c_int32 * 8), ("min_group_lanes", C.c_int32),
+                ("sweep_group", C.c_int32), ("group_requests", C.c_int32), ("pad", C.c_int32),
+                ("sweep_bound", C.c_int64),
+                ("resume_cap", C.c_int64)]
+
+    def as_dict(self):
+        return {f: ([int(v) for v in getattr(self, f)] if f == "draw_waves" else int(getattr(self, f)))
+                for f, _ in self._fields_ if f != "pad"}
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        for k, v in d.items():
+            if k == "draw_waves":
+                for i, w in enumerate(v):
+                    t.draw_waves[i] = int(w)
+            elif k != "pad":
+                setattr(t, k, int(v))
+        return t
 
 
 class DarkDesc(C.Structure):
@@ -148,6 +180,16 @@ def load():
     lib.petite_map_accumulate.restype = C.c_int
     lib.petite_set_grid.argtypes = [vp, i32, vp, i64]
     lib.petite_set_grid.restype = C.c_int
+    lib.petite_sample_scattering.argtypes = [vp, vp, vp, i64, u64, C.POINTER(Store), vp, vp]
+    lib.petite_sample_scattering.restype = C.c_int
+    lib.petite_propagate.argtypes = [vp, C.POINTER(Store), i64, u64, vp]
+    lib.petite_propagate.restype = C.c_int
+    lib.petite_draw_pe.argtypes = [vp, i64, u64, f64, vp, vp]
+    lib.petite_draw_pe.restype = C.c_int
+    lib.petite_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    lib.petite_set_tuning.restype = C.c_int
+    lib.petite_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    lib.petite_get_tuning.restype = C.c_int
     _lib = lib
     return lib
 

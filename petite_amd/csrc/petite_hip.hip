@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -36,7 +37,10 @@ int fail(int code, const std::string& msg) {
     if (_e != hipSuccess) return fail(PETITE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
   } while (0)
 
-enum StatIdx { ST_STEPS = 0, ST_DRAWS, ST_TRIALS, ST_VIOL, ST_NOSAMPLE, ST_DROPPED, ST_OVERFLOW, ST_N = 8 };
+// ST_OVERFLOW: children past the store capacity; ST_REQ_OVERFLOW: draw requests
+// past their process's region (impossible while a sweep holds at most as many
+// particles as a region holds requests; counted as a backstop).
+enum StatIdx { ST_STEPS = 0, ST_DRAWS, ST_TRIALS, ST_VIOL, ST_NOSAMPLE, ST_DROPPED, ST_OVERFLOW, ST_REQ_OVERFLOW, ST_N = 8 };
 
 // Layout of the context's counter block (unsigned long long).
 enum CounterIdx {
@@ -76,24 +80,25 @@ struct DrawLaunch {
   unsigned long long* counters;
   const unsigned int* dcount;  // non-null: counts on the device, plan made by each block
   int32_t full_blocks;         // blocks of a whole-GPU launch (device plan)
+  int32_t min_lg;              // log2 of the fewest lanes per request
+  int32_t group_requests;      // requests a lane group is sized to serve (small counts)
 };
 
 // Launch plan of one process's draw kernel: groups of 2^lg lanes share a
-// request, at least 4 (measured on MI355X: 10 % faster than single lanes, whose
-// refills - request loads, LU_Key search, max_F - come four times as often),
-// more when there are fewer than four requests per group, sized so the GPU
-// stays filled and each group sees a few requests.
-constexpr int kMinGroupLog = 2;
-__host__ __device__ inline void draw_plan(double cnt, int full_blocks, int& lg, int& blocks) {
+// request, at least 2^min_lg (default 4 lanes; measured on MI355X: 10 % faster
+// than single lanes, whose refills - request loads, LU_Key search, max_F -
+// come four times as often), more when there are fewer than `rpg` requests per
+// group, sized so the GPU stays filled and each group sees about rpg requests.
+__host__ __device__ inline void draw_plan(double cnt, int full_blocks, int min_lg, int rpg, int& lg, int& blocks) {
   const double full_lanes = 256.0 * full_blocks;
-  lg = kMinGroupLog;
+  lg = min_lg;
   blocks = 0;
   if (cnt <= 0.0) return;
-  if (cnt >= 4.0 * full_lanes) {
+  if (cnt >= (double)rpg * full_lanes) {
     blocks = full_blocks;
     return;
   }
-  const double want = 4.0 * full_lanes / cnt;
+  const double want = (double)rpg * full_lanes / cnt;
   while ((1 << lg) < want && lg < 6) ++lg;
   const double lanes = cnt * (double)(1 << lg) < full_lanes ? cnt * (double)(1 << lg) : full_lanes;
   blocks = (int)ceil(lanes / 256.0);
@@ -179,9 +184,24 @@ __device__ inline bool draw_possible(const DevTables& T) {
   return true;
 }
 
+// The float64 reference-order integrand of a 4-D process (see f_brem_ref).
+template <int P>
+__device__ inline double integrand_ref(const DevTables& T, double E, const double* x) {
+  if (P == BREM) return f_brem_ref(T, E, kMe, x[0], x[1], x[2], x[3]);
+  if (P == MUONBREM) return f_brem_ref(T, E, kMmu, x[0], x[1], x[2], x[3]);
+  if (P == PAIRPROD) return f_pairprod_ref(T, E, x[0], x[1], x[2], x[3]);
+  return integrand_p<P>(T, E, x);
+}
+
+// Relative band around the accept threshold max_F * u in which a float32
+// J*f of a 4-D process is re-evaluated in float64: the float32 bodies agree
+// with the float64 reference formula to 5e-6 (99th percentile) and 2e-5
+// (99.9th), so outside the band the two make the same decision.
+constexpr double kRefBand = 4.0e-4;
+
 template <int P, int ND>
 __device__ inline double candidate_w(const DevTables& T, double E, const float* base, const int32_t* ninc,
-                                     uint64_t seed, uint64_t sid, uint32_t j, double* x, double& ua) {
+                                     uint64_t seed, uint64_t sid, uint32_t j, double* x, double& ua, double& J) {
   const U4 a = draw_block(seed, sid, j, PURPOSE_VEGAS_A);
   const uint32_t rr[4] = {a.x, a.y, a.z, a.w};
   if (ND == 4) {
@@ -191,7 +211,7 @@ __device__ inline double candidate_w(const DevTables& T, double E, const float* 
   } else {
     ua = u01d(a.w);
   }
-  const double J = map_point_n<ND>(base, ninc, rr, x);
+  J = map_point_n<ND>(base, ninc, rr, x);
   return J * integrand_p<P>(T, E, x);
 }
 
@@ -261,9 +281,13 @@ __device__ void draw_loop(const DevTables& T, const DrawLaunch& io, uint64_t see
     }
     if (__ballot(have) == 0ull) break;
     double x[4] = {0.0, 0.0, 0.0, 0.0};
-    double w = 0.0, ua = 1.0;
+    double w = 0.0, ua = 1.0, J = 0.0;
     const uint32_t jc = j + (uint32_t)gl;
-    if (have) w = candidate_w<p, ND>(T, E, base, ninc, seed, sid, jc, x, ua);
+    if (have) {
+      w = candidate_w<p, ND>(T, E, base, ninc, seed, sid, jc, x, ua, J);
+      // near the threshold the decision is the float64 reference formula's
+      if (ND == 4 && fabs(mf * ua - w) <= kRefBand * w) w = J * integrand_ref<p>(T, E, x);
+    }
     const bool acc = have && (jc < max_trials) && (mf * ua < w);
     const unsigned long long am = __ballot(acc) & gmask;
     if (have) {
@@ -311,9 +335,11 @@ __device__ void draw_loop(const DevTables& T, const DrawLaunch& io, uint64_t see
 template <int P>
 __global__ __launch_bounds__(256) void k_draw(DevTables T, DrawLaunch io, uint64_t seed) {
   if (io.dcount) {
-    const int64_t cnt = io.dcount[P];
+    // a count past the region (a backstop: the transport never appends one)
+    // is cut to the region; the overflow is reported by the run
+    const int64_t cnt = min((int64_t)io.dcount[P], io.req_cap);
     int lg, blocks;
-    draw_plan((double)cnt, io.full_blocks, lg, blocks);
+    draw_plan((double)cnt, io.full_blocks, io.min_lg, io.group_requests, lg, blocks);
     if ((int)blockIdx.x >= blocks) return;
     draw_loop<P>(T, io, seed, cnt, lg);
   } else {
@@ -637,16 +663,18 @@ __device__ inline void charged_advance(const DevTables& T, double dist, double m
 // for the store slots [first, first + n).  One advance per loop iteration.
 // Refills (record loads, photon flights) and finishes (record stores, process
 // choice, draw requests) run for the whole wave at once, so they wait until
-// kBatchLanes lanes need them (or no lane is stepping): each costs about as
+// batch_lanes lanes need them (or no lane is stepping): each costs about as
 // much as several substeps (global loads, atomics), and measured on MI355X
 // 44 of 64 lanes gives the shortest sweeps (32 and 56: 1.13x longer).
-constexpr int kBatchLanes = 44;
-// A particle that has taken kYieldSteps substeps in one launch hands its live
+// A particle that has taken yield_steps substeps in one launch hands its live
 // state on to the next sweep, which starts it first: the few long-lived
 // particles otherwise keep nearly empty waves running at the end of every
 // launch.  Its Philox blocks are indexed by its total substep count, so the
 // random numbers, and every result, are those of an uninterrupted transport.
-constexpr uint32_t kYieldSteps = 40;
+struct TransportParams {
+  int32_t batch_lanes;   // 1..64
+  uint32_t yield_steps;  // >= 1
+};
 struct ResumeRec {
   double E, dep, dz;
   float nx, ny, nz, rx, ry, rz;
@@ -662,7 +690,7 @@ struct ResumeBuf {
 __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int64_t first, int64_t n, uint64_t seed,
                                                    Requests R, ParentRecs PR, double* shower_energy,
                                                    unsigned long long* counters, const double* sig_lds_src,
-                                                   const int64_t* range, ResumeBuf RB) {
+                                                   const int64_t* range, ResumeBuf RB, TransportParams TP) {
   // resumed particles (the previous sweep's hand-ons) come first in the queue
   int64_t nres = 0;
   const ResumeRec* rin = nullptr;
@@ -695,13 +723,13 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
   uint32_t k = 0;
   uint32_t kl = 0;       // substeps taken in this launch
   bool closing = false;  // the next advance is the closing increment
-  unsigned long long steps = 0;
+  unsigned long long steps = 0, req_ovf = 0;
   while (true) {
     const unsigned long long am = __ballot(act);
     // ---- refill empty lanes in batches
     const bool idle = !act && !wait && !done;
     const unsigned long long im = __ballot(idle);
-    if (im && (__popcll(im) >= kBatchLanes || am == 0ull)) {
+    if (im && (__popcll(im) >= TP.batch_lanes || am == 0ull)) {
       const int64_t r = q.take(im, idle, counters + C_TRANSPORT);
       if (idle) {
         if (r < nres) {
@@ -786,7 +814,7 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
     }
     // ---- finish parked particles in batches: write pf/rf, choose the process, queue the draw
     const unsigned long long wm = __ballot(wait);
-    if (wm && (__popcll(wm) >= kBatchLanes || __ballot(act) == 0ull)) {
+    if (wm && (__popcll(wm) >= TP.batch_lanes || __ballot(act) == 0ull)) {
       int want = -1;
       if (wait) {
         steps += k;
@@ -840,6 +868,17 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
         if (want >= 0) {
           const unsigned idx = base + (unsigned)__popcll(mine & ((1ull << lane) - 1ull));
           const int64_t o = (int64_t)want * R.cap + idx;
+          // a region holds as many requests as a sweep holds particles, so
+          // this never fails; past the region the request is dropped and
+          // counted, and the run reports an internal error
+          if ((int64_t)idx >= R.cap) {
+            ++req_ovf;
+            want = -1;
+          }
+        }
+        if (want >= 0) {
+          const unsigned idx = base + (unsigned)__popcll(mine & ((1ull << lane) - 1ull));
+          const int64_t o = (int64_t)want * R.cap + idx;
           R.slot[o] = (int32_t)slot;
           R.E[o] = E;
           PR.sid[o] = sid;
@@ -877,7 +916,12 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
         dist = dz;
         w_r = u.z;
         w_a = u.w;
-        if (u01d(u.x) > (double)__expf(-1.0f / (float)div)) {
+        // hard if u > exp(-dz/mfp): the float32 exponential decides unless u
+        // lies within 1e-5 of it (its error is a few 1e-7), where float64 does
+        const double ux = u01d(u.x);
+        double thr = (double)__expf(-1.0f / (float)div);
+        if (fabs(ux - thr) < 1.0e-5) thr = exp(-(dz / mfp));
+        if (ux > thr) {
           adv = false;
           closing = true;
         }
@@ -904,7 +948,7 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
         closing = true;  // below the cutoff after this substep: close next
       }
       // hand a long-lived particle on to the next sweep (space permitting)
-      const bool give = rout && !closing && act && ++kl >= kYieldSteps;
+      const bool give = rout && !closing && act && ++kl >= TP.yield_steps;
       const unsigned long long gm = __ballot(give);
       if (gm) {
         const int leader = __ffsll((long long)gm) - 1;
@@ -931,9 +975,41 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
     }
   }
   wave_stat(counters, C_STATS + ST_STEPS, steps);
+  wave_stat(counters, C_STATS + ST_REQ_OVERFLOW, req_ovf);
 }
 
 // -------------------------------------------------------------- emission --
+// The two children of one vertex in the lab (sample_scattering,
+// shower.py:471-480): the process's final state in the parent frame from the
+// accepted point xs and the azimuth uniform az, rotated onto the parent's
+// direction (px, py, pz; any length), and their species (process_PIDS, 87-96).
+__device__ inline void vertex_children(int p, int pid, double E, const float* xs, double az, double px, double py,
+                                       double pz, V4* lab, int* cpid) {
+  const double m = pid_mass(pid);
+  V4 a, b;
+  switch (p) {
+    case BREM: case MUONBREM: kin_brem(E, m, xs, az, a, b); break;
+    case PAIRPROD: kin_pairprod(E, xs, az, a, b); break;
+    case COMP: kin_compton(E, 0.0, (double)xs[0], az, a, b); break;
+    case ANN: kin_ann(E, 0.0, (double)xs[0], az, a, b); break;
+    case MUONE: kin_mue(E, (double)xs[0], az, a, b); break;
+    default: kin_ee(E, (double)xs[0], az, a, b); break;  // Moller, Bhabha
+  }
+  child_pids(p, pid, &cpid[0], &cpid[1]);
+  const double pn = sqrt(px * px + py * py + pz * pz);
+  if (pn > 0.0) {
+    px /= pn;
+    py /= pn;
+    pz /= pn;
+  }
+  const V4 in[2] = {a, b};
+  for (int k = 0; k < 2; ++k) {
+    double ox, oy, oz;
+    frame_apply_d(px, py, pz, in[k], ox, oy, oz);
+    lab[k] = V4{in[k].e, ox, oy, oz};
+  }
+}
+
 // sample_scattering's kinematics, lab rotation and child bookkeeping
 // (shower.py:467-507, 702-706) for every finished draw.
 struct EmitLaunch {
@@ -955,7 +1031,7 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
   int64_t seg[PETITE_NPROC + 1];
   seg[0] = 0;
 #pragma unroll
-  for (int q = 0; q < PETITE_NPROC; ++q) seg[q + 1] = seg[q] + (int64_t)io.dcount[q];
+  for (int q = 0; q < PETITE_NPROC; ++q) seg[q + 1] = seg[q] + min((int64_t)io.dcount[q], io.req_cap);
   const int64_t total = seg[PETITE_NPROC];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t r0 = (int64_t)blockIdx.x * blockDim.x; r0 < total; r0 += stride) {
@@ -991,30 +1067,9 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
       const float* xs = io.x_out + 4 * oidx;
       sid = io.par.sid[oidx];
       const double az = u01d(draw_block(seed, sid, 0, PURPOSE_INTERACT).y);
-      const double m = pid_mass(pid);
-      V4 a, b;
-      switch (p) {
-        case BREM: case MUONBREM: kin_brem(E, m, xs, az, a, b); break;
-        case PAIRPROD: kin_pairprod(E, xs, az, a, b); break;
-        case COMP: kin_compton(E, 0.0, (double)xs[0], az, a, b); break;
-        case ANN: kin_ann(E, 0.0, (double)xs[0], az, a, b); break;
-        case MUONE: kin_mue(E, (double)xs[0], az, a, b); break;
-        default: kin_ee(E, (double)xs[0], az, a, b); break;  // Moller, Bhabha
-      }
-      child_pids(p, pid, &cpid[0], &cpid[1]);
       const float4 pf = io.par.pf[oidx];
-      double nx = pf.y, ny = pf.z, nz = pf.w;
-      const double pn = sqrt(nx * nx + ny * ny + nz * nz);
-      if (pn > 0.0) {
-        nx /= pn;
-        ny /= pn;
-        nz /= pn;
-      }
-      const V4 in[2] = {a, b};
+      vertex_children(p, pid, E, xs, az, pf.y, pf.z, pf.w, lab, cpid);
       for (int k = 0; k < 2; ++k) {
-        double ox, oy, oz;
-        frame_apply_d(nx, ny, nz, in[k], ox, oy, oz);
-        lab[k] = V4{in[k].e, ox, oy, oz};
         keep[k] = lab[k].e > T.d->min_energy;
         if (!keep[k] && shower_energy) unsafeAtomicAdd(shower_energy + 4 * shower + 2, lab[k].e);
       }
@@ -1062,6 +1117,57 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
   }
 }
 
+// ------------------------------------------------------- single vertices --
+// sample_scattering for given particles and processes (shower.py:467-507):
+// requests into the per-process regions (slot = the vertex index, stream id
+// beside it), then, after the draws, both children of every vertex.
+__global__ void k_vertex_requests(DevTables T, const PetiteVertex* v, const int32_t* proc, int64_t n, Requests R,
+                                  uint64_t* req_sid, int32_t* trials_out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int p = proc[i];
+  if (p < 0 || p >= kNprocSM || T.d->grid[p].n_E <= 0) {
+    trials_out[i] = 0;
+    return;
+  }
+  const unsigned k = atomicAdd(R.count + p, 1u);
+  const int64_t o = (int64_t)p * R.cap + k;
+  R.slot[o] = (int32_t)i;
+  R.E[o] = v[i].E;
+  req_sid[o] = v[i].sid;
+}
+
+// Children 2i and 2i+1 of vertex i (child index k = 0, 1), every child kept
+// (sample_scattering returns both; generate_shower's cut is the caller's).
+__global__ void k_vertex_emit(const PetiteVertex* v, const int32_t* proc, int64_t n, const float* x_out,
+                              const int32_t* trials, uint64_t seed, DevStore out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const PetiteVertex q = v[i];
+  const int p = proc[i];
+  const bool ok = trials[i] > 0;
+  V4 lab[2] = {V4{0.0, 0.0, 0.0, 0.0}, V4{0.0, 0.0, 0.0, 0.0}};
+  int cpid[2] = {0, 0};
+  if (ok) {
+    const double az = u01d(draw_block(seed, q.sid, 0, PURPOSE_INTERACT).y);
+    vertex_children(p, q.pid, q.E, x_out + 4 * i, az, q.px, q.py, q.pz, lab, cpid);
+  }
+  for (int k = 0; k < 2; ++k) {
+    const int64_t c = 2 * i + k;
+    const float4 p4 = make_float4((float)lab[k].e, (float)lab[k].x, (float)lab[k].y, (float)lab[k].z);
+    const float4 r4 = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.weight);
+    out.p0[c] = p4;
+    out.pf[c] = p4;
+    out.r0[c] = r4;
+    out.rf[c] = make_float4((float)q.x, (float)q.y, (float)q.z, 0.0f);
+    out.pid[c] = cpid[k];
+    out.parent[c] = (int32_t)i;
+    out.meta[c] = ok ? (((q.generation + 1) & 0xFFFF) | ((p + 1) << 16) | (k << 24)) : 0;
+    out.shower[c] = 0;
+    out.sid[c] = derive_sid(seed, q.sid, (uint32_t)k, PURPOSE_CHILD);
+  }
+}
+
 // Start of a device-driven sweep: the range is the particles appended since
 // the last sweep (at most `bound`, the rest waits for the next sweep), and the
 // per-sweep counters are cleared.  range = {first, n, end}.
@@ -1071,19 +1177,21 @@ __global__ void k_sweep_setup(unsigned long long* ctr, int64_t* range, unsigned 
   if (t < PETITE_NPROC) req_count[t] = 0u;
   if (t <= C_TRANSPORT) ctr[t] = 0ull;  // draw refill and transport refill counters
   if (t == 0) {
-    const int64_t first = range[2];
-    int64_t now = (int64_t)ctr[C_STORE];
-    if (now > capacity) now = capacity;
-    int64_t n = now - first;
-    if (n > bound) n = bound;
-    if (n < 0) n = 0;
-    range[0] = first;
-    range[1] = n;
-    range[2] = first + n;
     // the previous transport's hand-ons; a reservation past the buffer kept
     // its particle stepping, so only the first resume_cap records exist
     int64_t nres = (int64_t)ctr[C_YIELD_OUT];
     if (nres > resume_cap) nres = resume_cap;
+    // new particles fill the sweep up to `bound` with the resumed ones, so
+    // that no process gets more draw requests than its region holds
+    const int64_t first = range[2];
+    int64_t now = (int64_t)ctr[C_STORE];
+    if (now > capacity) now = capacity;
+    int64_t n = now - first;
+    if (n > bound - nres) n = bound - nres;
+    if (n < 0) n = 0;
+    range[0] = first;
+    range[1] = n;
+    range[2] = first + n;
     ctr[C_YIELD_OUT] = 0ull;
     ctr[C_RPARITY] ^= 1ull;
     range[3] = nres;
@@ -1092,6 +1200,17 @@ __global__ void k_sweep_setup(unsigned long long* ctr, int64_t* range, unsigned 
 }
 
 #include "dark.h"
+
+// draw_pe_sample (dark_shower.py:706-719) for n stream ids: the bound
+// electron's momentum [GeV], or -1 when no sample came within 65536 pairs.
+__global__ void k_draw_pe(const uint64_t* sid, int64_t n, uint64_t seed, double Zeff, double* pe_out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double pe = -1.0;
+  if (!dark::draw_pe(seed, sid[i], Zeff, pe)) pe = -1.0;
+  pe_out[i] = pe;
+}
+
 
 // ------------------------------------------------- detector acceptance --
 // transverse_position and detector_cut (shower.py:815-865): straight-line
@@ -1228,6 +1347,7 @@ struct PetiteCtx {
   int64_t resume_cap = 0;
   std::vector<hipEvent_t> tev;
   int n_cu = 256;
+  PetiteTuning tune{};  // launch and queue sizes in effect (defaults filled in)
   double tbar[PETITE_NPROC];  // running mean of candidates per draw, per process
   // dark sector
   bool has_dark = false;
@@ -1274,10 +1394,39 @@ static int ensure_entries(PetiteCtx* c, int64_t cap) {
   return PETITE_OK;
 }
 
-// Device-driven sweeps: at most kSweepBound particles per sweep (request
-// scratch scales with it) and kSweepGroup sweeps queued per host check.
-constexpr int64_t kSweepBound = (int64_t)1 << 23;
-constexpr int kSweepGroup = 16;
+// Default launch and queue sizes (PetiteTuning), measured on MI355X with the
+// 100k-primary lead workload.  Device-driven sweeps: at most 2^23 particles
+// per sweep (request scratch scales with it) and 16 sweeps queued per host
+// check.
+static PetiteTuning default_tuning() {
+  PetiteTuning t{};
+  t.batch_lanes = 44;
+  t.yield_steps = 40;
+  t.transport_waves = 20;
+  t.emit_waves = 32;
+  for (int p = 0; p < 8; ++p) t.draw_waves[p] = 2;
+  t.draw_waves[BREM] = 16;
+  t.draw_waves[PAIRPROD] = 8;
+  t.min_group_lanes = 4;
+  t.sweep_group = 16;
+  t.group_requests = 4;
+  t.sweep_bound = (int64_t)1 << 23;
+  t.resume_cap = (int64_t)1 << 20;
+  return t;
+}
+
+// log2 of a power-of-two lane count.
+static int lanes_log2(int lanes) {
+  int lg = 0;
+  while ((1 << lg) < lanes && lg < 6) ++lg;
+  return lg;
+}
+
+// Blocks of 4 waves for `waves` waves per CU (at least one block).
+static unsigned blocks_for(const PetiteCtx* c, int waves) {
+  const int64_t b = ((int64_t)c->n_cu * waves + 3) / 4;
+  return (unsigned)std::max<int64_t>(1, b);
+}
 
 static int ensure_requests(PetiteCtx* c, int64_t cap) {
   if (cap <= c->req_cap) return PETITE_OK;
@@ -1320,10 +1469,12 @@ static int ensure_parent_recs(PetiteCtx* c) {
 static void plan_draw(PetiteCtx* c, const unsigned int* rq, DrawLaunch& io, int64_t& blocks_total) {
   io.block_start[0] = 0;
   io.full_blocks = c->n_cu * 4;  // 4-wave blocks at 4 waves per SIMD
+  io.min_lg = lanes_log2(c->tune.min_group_lanes);
+  io.group_requests = c->tune.group_requests;
   for (int p = 0; p < PETITE_NPROC; ++p) {
     int b, lg;
     io.count[p] = (int32_t)rq[p];
-    draw_plan((double)rq[p], io.full_blocks, lg, b);
+    draw_plan((double)rq[p], io.full_blocks, io.min_lg, io.group_requests, lg, b);
     io.lgG[p] = lg;
     io.block_start[p + 1] = io.block_start[p] + b;
   }
@@ -1379,13 +1530,7 @@ static int ensure_draw_streams(PetiteCtx* c) {
 // others finish early, after which Brem runs alone at full occupancy
 // (4 blocks of 4 waves per CU).  Measured on MI355X against n_cu * 2 blocks
 // for every process: draws 13 % shorter.
-static int draw_blocks(const PetiteCtx* c, int p) {
-  switch (p) {
-    case BREM: return c->n_cu * 4;
-    case PAIRPROD: return c->n_cu * 2;
-    default: return c->n_cu / 2;
-  }
-}
+static int draw_blocks(const PetiteCtx* c, int p) { return (int)blocks_for(c, c->tune.draw_waves[p & 7]); }
 
 static int launch_draws_device(PetiteCtx* c, const DrawLaunch& io, uint64_t seed, hipStream_t s, bool muons) {
   int rc = ensure_draw_streams(c);
@@ -1464,6 +1609,7 @@ int petite_ctx_create(const PetiteTableDesc* desc, const double* blob64, int64_t
   T.brem_C_e = (float)Ce;
   T.brem_C_mu = (float)Cmu;
   T.a0sq = (float)(a0 * a0);
+  T.brem_a0 = a0;
   for (int p = 0; p < PETITE_NPROC; ++p) c->tbar[p] = 16.0;
   T.mcs_cA = (float)(0.157 * Z * (Z + 1.0) / desc->A);
   T.mcs_cB = (float)(2.007e-5 * pow(Z, 2.0 / 3.0));
@@ -1500,6 +1646,7 @@ int petite_ctx_create(const PetiteTableDesc* desc, const double* blob64, int64_t
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
     c->n_cu = prop.multiProcessorCount;
+  c->tune = default_tuning();
   *out = c;
   return PETITE_OK;
 }
@@ -1525,6 +1672,47 @@ void petite_ctx_destroy(PetiteCtx* c) {
     if (p) (void)hipFree(p);
   free_entries(c);
   delete c;
+}
+
+int petite_set_tuning(PetiteCtx* c, const PetiteTuning* in) {
+  if (!c) return fail(PETITE_ERR_ARG, "petite_set_tuning: null context");
+  const PetiteTuning d = default_tuning();
+  if (!in) {
+    c->tune = d;
+    return PETITE_OK;
+  }
+  PetiteTuning t = *in;
+  auto pick = [](auto v, auto def) { return v > 0 ? v : def; };
+  t.batch_lanes = pick(t.batch_lanes, d.batch_lanes);
+  t.yield_steps = pick(t.yield_steps, d.yield_steps);
+  t.transport_waves = pick(t.transport_waves, d.transport_waves);
+  t.emit_waves = pick(t.emit_waves, d.emit_waves);
+  for (int p = 0; p < 8; ++p) t.draw_waves[p] = pick(t.draw_waves[p], d.draw_waves[p]);
+  t.min_group_lanes = pick(t.min_group_lanes, d.min_group_lanes);
+  t.sweep_group = pick(t.sweep_group, d.sweep_group);
+  t.group_requests = pick(t.group_requests, d.group_requests);
+  t.sweep_bound = pick(t.sweep_bound, d.sweep_bound);
+  t.resume_cap = pick(t.resume_cap, std::min<int64_t>(d.resume_cap, t.sweep_bound));
+  t.pad = 0;
+  if (t.batch_lanes > 64) return fail(PETITE_ERR_ARG, "petite_set_tuning: batch_lanes above 64");
+  if (t.transport_waves > 64 || t.emit_waves > 64) return fail(PETITE_ERR_ARG, "petite_set_tuning: more than 64 waves per CU");
+  for (int p = 0; p < 8; ++p)
+    if (t.draw_waves[p] > 64) return fail(PETITE_ERR_ARG, "petite_set_tuning: more than 64 draw waves per CU");
+  if (t.min_group_lanes > 64 || (t.min_group_lanes & (t.min_group_lanes - 1)) != 0)
+    return fail(PETITE_ERR_ARG, "petite_set_tuning: min_group_lanes is not a power of two up to 64");
+  if (t.sweep_group > 64) return fail(PETITE_ERR_ARG, "petite_set_tuning: sweep_group above 64");
+  if (t.group_requests > 16) return fail(PETITE_ERR_ARG, "petite_set_tuning: group_requests above 16");
+  if (t.sweep_bound < 64 || t.sweep_bound > ((int64_t)1 << 26))
+    return fail(PETITE_ERR_ARG, "petite_set_tuning: sweep_bound outside 64..2^26");
+  if (t.resume_cap > t.sweep_bound) return fail(PETITE_ERR_ARG, "petite_set_tuning: resume_cap above sweep_bound");
+  c->tune = t;
+  return PETITE_OK;
+}
+
+int petite_get_tuning(const PetiteCtx* c, PetiteTuning* t) {
+  if (!c || !t) return fail(PETITE_ERR_ARG, "petite_get_tuning: bad arguments");
+  *t = c->tune;
+  return PETITE_OK;
 }
 
 int petite_set_timing(PetiteCtx* c, int32_t on) {
@@ -1648,23 +1836,24 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   std::memset(stats, 0, sizeof(*stats));
   // A sweep transports the particles appended since the previous one (at
   // most `bound` of them), draws their interactions and appends the children.
-  // Sizes live on the device: the host queues kSweepGroup sweeps at a time
-  // and reads the counters once per group.
-  const int64_t bound = std::min<int64_t>(store->capacity, kSweepBound);
+  // Sizes live on the device: the host queues tune.sweep_group sweeps at a
+  // time and reads the counters once per group.
+  const PetiteTuning& tn = c->tune;
+  const int64_t bound = std::min<int64_t>(store->capacity, tn.sweep_bound);
   int rc = ensure_requests(c, bound);
   if (rc) return rc;
   rc = ensure_parent_recs(c);
   if (rc) return rc;
   if (!c->drange) HIP_TRY(hipMalloc(&c->drange, 4 * sizeof(int64_t)));
-  {
-    const int64_t want = std::min<int64_t>(bound, (int64_t)1 << 20);
-    if (c->resume_cap < want) {
-      if (c->dresume) HIP_TRY(hipFree(c->dresume));
-      c->dresume = nullptr;
-      c->resume_cap = 0;
-      HIP_TRY(hipMalloc(&c->dresume, 2 * want * sizeof(ResumeRec)));
-      c->resume_cap = want;
-    }
+  // hand-on records per sweep: the buffer keeps its allocation, the cap in
+  // use is the tuning's (at most the sweep bound)
+  const int64_t rcap = std::max<int64_t>(1, std::min<int64_t>(bound, tn.resume_cap));
+  if (c->resume_cap < rcap) {
+    if (c->dresume) HIP_TRY(hipFree(c->dresume));
+    c->dresume = nullptr;
+    c->resume_cap = 0;
+    HIP_TRY(hipMalloc(&c->dresume, 2 * rcap * sizeof(ResumeRec)));
+    c->resume_cap = rcap;
   }
   unsigned long long* ctr = c->counters;
   HIP_TRY(hipMemsetAsync(ctr, 0, C_N * sizeof(unsigned long long), s));
@@ -1700,11 +1889,15 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   em.trials_out = c->res_trials;
   em.dcount = c->req_count;
   em.par = c->par;
-  const unsigned tr_blocks = (unsigned)(c->n_cu * 5);  // 20 waves per CU
-  const unsigned em_blocks = (unsigned)(c->n_cu * 8);
+  io.min_lg = lanes_log2(tn.min_group_lanes);
+  io.group_requests = tn.group_requests;
+  const unsigned tr_blocks = blocks_for(c, tn.transport_waves);
+  const unsigned em_blocks = blocks_for(c, tn.emit_waves);
+  const TransportParams TP{tn.batch_lanes, (uint32_t)tn.yield_steps};
+  const int group = tn.sweep_group;
   const size_t lds = (size_t)c->dev.lds_doubles * sizeof(double);
   if (c->timing) {
-    while ((int)c->tev.size() < 4 * kSweepGroup) {
+    while ((int)c->tev.size() < 4 * group) {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
       c->tev.push_back(e);
@@ -1715,14 +1908,14 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   unsigned long long h_store = n0;
   int64_t h_range[4] = {0, 0, 0, 0};
   while (true) {
-    for (int g = 0; g < kSweepGroup; ++g) {
+    for (int g = 0; g < group; ++g) {
       hipLaunchKernelGGL(k_sweep_setup, dim3(1), dim3(64), 0, s, ctr, c->drange, c->req_count, store->capacity, bound,
-                         c->resume_cap);
+                         rcap);
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 0], s));
       hipLaunchKernelGGL(k_transport, dim3(tr_blocks), dim3(256), lds, s, c->dev, S, (int64_t)0, (int64_t)0, seed, R,
                          c->par, shower_energy, ctr, c->dsig_lds, (const int64_t*)c->drange,
-                         ResumeBuf{c->dresume, c->resume_cap});
+                         ResumeBuf{c->dresume, rcap}, TP);
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 1], s));
       rc = launch_draws_device(c, io, seed, s, muons);
@@ -1733,12 +1926,12 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 3], s));
     }
-    queued += kSweepGroup;
+    queued += group;
     HIP_TRY(hipMemcpyAsync(&h_store, ctr + C_STORE, sizeof(h_store), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(h_range, c->drange, sizeof(h_range), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (c->timing) {
-      for (int g = 0; g < kSweepGroup; ++g) {
+      for (int g = 0; g < group; ++g) {
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, c->tev[4 * g + 0], c->tev[4 * g + 1]));
         t_tr += ms;
@@ -1783,9 +1976,96 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   stats->launches_transport = queued;
   stats->launches_draw = queued;
   stats->launches_emit = queued;
+  if (hs[ST_REQ_OVERFLOW])
+    return fail(PETITE_ERR_INTERNAL, "petite_run_showers: draw requests overflowed their region (library bug)");
   if (written > store->capacity)
     return fail(PETITE_ERR_CAPACITY, "petite_run_showers: store capacity exceeded; n_particles holds the size reached");
   if (hs[ST_NOSAMPLE]) return fail(PETITE_ERR_NO_SAMPLE, "petite_run_showers: No Sample Found within max_trials");
+  return PETITE_OK;
+}
+
+int petite_sample_scattering(PetiteCtx* c, const PetiteVertex* v, const int32_t* proc, int64_t n, uint64_t seed,
+                             PetiteStore* out, int32_t* trials_out, PetiteStream stream) {
+  if (!c || n < 0 || (n > 0 && (!v || !proc || !out || !trials_out)))
+    return fail(PETITE_ERR_ARG, "petite_sample_scattering: bad arguments");
+  if (n == 0) return PETITE_OK;
+  if (n > INT32_MAX / 2) return fail(PETITE_ERR_ARG, "petite_sample_scattering: more than 2^30 vertices");
+  if (out->capacity < 2 * n) return fail(PETITE_ERR_CAPACITY, "petite_sample_scattering: out holds fewer than 2n entries");
+  hipStream_t s = (hipStream_t)stream;
+  int rc = ensure_requests(c, n);
+  if (rc) return rc;
+  rc = ensure_parent_recs(c);
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(c->counters, 0, C_N * sizeof(unsigned long long), s));
+  HIP_TRY(hipMemsetAsync(c->req_count, 0, PETITE_NPROC * sizeof(unsigned int), s));
+  Requests R{c->req_slot, c->req_E, c->req_cap, c->req_count};
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_vertex_requests, dim3(nb), dim3(256), 0, s, c->dev, v, proc, n, R, c->par.sid, trials_out);
+  HIP_TRY(hipGetLastError());
+  unsigned int rq[PETITE_NPROC];
+  HIP_TRY(hipMemcpyAsync(rq, c->req_count, sizeof(rq), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  DrawLaunch io{};
+  io.req_slot = c->req_slot;
+  io.req_E = c->req_E;
+  io.req_sid = c->par.sid;
+  io.req_cap = c->req_cap;
+  io.explicit_out = 1;
+  io.x_out = c->res_x;
+  io.trials_out = trials_out;
+  io.w_out = c->res_w;
+  io.counters = c->counters;
+  int64_t blocks = 0;
+  plan_draw(c, rq, io, blocks);
+  rc = launch_draws(c, io, seed, s);
+  if (rc) return rc;
+  DevStore O{(float4*)out->p0, (float4*)out->r0, (float4*)out->pf, (float4*)out->rf, out->pid, out->parent, out->meta,
+             out->shower, out->sid, out->capacity};
+  hipLaunchKernelGGL(k_vertex_emit, dim3(nb), dim3(256), 0, s, v, proc, n, (const float*)c->res_x,
+                     (const int32_t*)trials_out, seed, O);
+  HIP_TRY(hipGetLastError());
+  unsigned long long st[ST_N];
+  HIP_TRY(hipMemcpyAsync(st, c->counters + C_STATS, sizeof(st), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (st[ST_NOSAMPLE]) return fail(PETITE_ERR_NO_SAMPLE, "petite_sample_scattering: No Sample Found within max_trials");
+  return PETITE_OK;
+}
+
+int petite_propagate(PetiteCtx* c, PetiteStore* recs, int64_t n, uint64_t seed, PetiteStream stream) {
+  if (!c || !recs || n < 0) return fail(PETITE_ERR_ARG, "petite_propagate: bad arguments");
+  if (n == 0) return PETITE_OK;
+  if (n > recs->capacity || n > INT32_MAX) return fail(PETITE_ERR_ARG, "petite_propagate: n above the store capacity");
+  hipStream_t s = (hipStream_t)stream;
+  // the transport also queues the draw requests of the particles it
+  // finishes; they go to the context's regions and are not used
+  int rc = ensure_requests(c, n);
+  if (rc) return rc;
+  rc = ensure_parent_recs(c);
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(c->counters, 0, C_N * sizeof(unsigned long long), s));
+  HIP_TRY(hipMemsetAsync(c->req_count, 0, PETITE_NPROC * sizeof(unsigned int), s));
+  DevStore S{(float4*)recs->p0, (float4*)recs->r0, (float4*)recs->pf, (float4*)recs->rf, recs->pid, recs->parent,
+             recs->meta, recs->shower, recs->sid, recs->capacity};
+  Requests R{c->req_slot, c->req_E, c->req_cap, c->req_count};
+  const PetiteTuning& tn = c->tune;
+  // no hand-on (no resume buffer): every particle is transported to its end in this launch
+  const TransportParams TP{tn.batch_lanes, 0xFFFFFFFFu};
+  const int64_t waves = (n + 63) / 64;
+  const unsigned blocks = (unsigned)std::max<int64_t>(1, std::min<int64_t>((waves + 3) / 4, blocks_for(c, tn.transport_waves)));
+  const size_t lds = (size_t)c->dev.lds_doubles * sizeof(double);
+  hipLaunchKernelGGL(k_transport, dim3(blocks), dim3(256), lds, s, c->dev, S, (int64_t)0, n, seed, R, c->par,
+                     (double*)nullptr, c->counters, c->dsig_lds, (const int64_t*)nullptr, ResumeBuf{nullptr, 0}, TP);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(s));
+  return PETITE_OK;
+}
+
+int petite_draw_pe(const uint64_t* sid, int64_t n, uint64_t seed, double Zeff, double* pe_out, PetiteStream stream) {
+  if (n < 0 || (n > 0 && (!sid || !pe_out)) || !(Zeff > 0.0)) return fail(PETITE_ERR_ARG, "petite_draw_pe: bad arguments");
+  if (n == 0) return PETITE_OK;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_draw_pe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, sid, n, seed, Zeff, pe_out);
+  HIP_TRY(hipGetLastError());
   return PETITE_OK;
 }
 
@@ -1999,12 +2279,19 @@ int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t
     return fail(PETITE_ERR_ARG, "petite_dark_yield: bad arguments");
   if (c->n_mass < 1) return fail(PETITE_ERR_ARG, "petite_dark_yield: no trained masses (petite_yield_setup)");
   if (n_sm > sm->capacity) return fail(PETITE_ERR_ARG, "petite_dark_yield: n_sm above the store capacity");
-  for (int k = 0; k < n_points; ++k)
+  const int nm = c->n_mass;
+  for (int k = 0; k < n_points; ++k) {
     if (!(mV[k] > 0.0)) return fail(PETITE_ERR_ARG, "petite_dark_yield: m_V must be positive");
+    if (mV[k] < c->masses[0] || mV[k] > c->masses[nm - 1]) {
+      char msg[200];
+      std::snprintf(msg, sizeof(msg), "petite_dark_yield: m_V = %.6g GeV outside the trained masses [%.6g, %.6g] GeV",
+                    mV[k], c->masses[0], c->masses[nm - 1]);
+      return fail(PETITE_ERR_ARG, msg);
+    }
+  }
   hipStream_t s = (hipStream_t)stream;
   DevStore S{(float4*)sm->p0, (float4*)sm->r0, (float4*)sm->pf, (float4*)sm->rf, sm->pid, sm->parent, sm->meta,
              sm->shower, sm->sid, sm->capacity};
-  const int nm = c->n_mass;
   const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n_sm + 255) / 256, (int64_t)c->n_cu * 8));
   // passes over the points in ascending m_V, so that a pass needs few trained masses
   std::vector<int> order(n_points);
@@ -2022,14 +2309,8 @@ int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t
       const double dl = std::log(c->masses[j + 1] / c->masses[j]);
       t = std::log(m / c->masses[j]) / dl;
       dtdm = 1.0 / (m * dl);
-      if (t <= 0.0) {
-        t = 0.0;
-        if (m < c->masses[0]) dtdm = 0.0;
-      }
-      if (t >= 1.0) {
-        t = 1.0;
-        if (m > c->masses[nm - 1]) dtdm = 0.0;
-      }
+      // inside the trained range (checked above) t leaves [0, 1] only by rounding
+      t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
     }
     seg[q] = j;
     tt[q] = t;

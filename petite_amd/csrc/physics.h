@@ -36,6 +36,7 @@ struct DevTables {
   float brem_C_e;      // 8/pi alpha (alpha/m)^2 m^4 Z^2 a0^4 for m = m_e
   float brem_C_mu;     // same for m = m_mu (a0 keeps m_e, all_processes.py:92-103)
   float a0sq;          // a0^2 [GeV^-2]
+  double brem_a0;      // a0 = 184.15 e^-1/2 Z^-1/3 / m_e (all_processes.py:92-94), float64
   float mcs_cA;        // Lynch-Dahl constants of the material: 0.157 Z (Z+1) / A,
   float mcs_cB;        //   2.007e-5 Z^(2/3),
   float mcs_cC;        //   3.34 (Z alpha)^2  (moliere.py:196-219)
@@ -234,6 +235,71 @@ __device__ inline float f_pairprod(const DevTables& T, double w, const double* x
   const float bracket = t1 * t1 + (fepp * fepp + fepm * fepm) / (2.0f * fepp * fepm) * i1 * i2 *
                                       (dpm * dpm + 4.0f * fdp * fdm * c2 * c2);
   return pfff * bracket * jac;
+}
+
+// The Brem and PairProd integrands in float64 with the reference's own term
+// order (all_processes.py:139-206, 531-622): T1+T2+T3+T4 summed directly and
+// the form factor g2_elastic (99-103) applied as a separate factor.  The
+// draws take these only for a candidate whose float32 J*f lies within a
+// small band of the accept threshold, so the accept decision is the one a
+// float64 evaluation of the reference formula makes.  Not inlined (and the
+// point passed by value): the rare call keeps the draw loop's registers at
+// the float32 body's count.
+__device__ __noinline__ double f_brem_ref(const DevTables& T, double ep, double m, double x0, double x1, double x2,
+                                          double x3) {
+  const double x[4] = {x0, x1, x2, x3};
+  const double egmin = T.d->Eg_min;
+  const double w = egmin + x[0] * (ep - m - egmin);
+  const double d = ep / (2.0 * m) * (x[1] + x[2]);
+  const double dp = ep / (2.0 * m) * (x[1] - x[2]);
+  const double ph = (x[3] - 0.5) * 2.0 * kPi;
+  const double epp = ep - w;
+  if (!((egmin < w) && (w < ep - m) && (m < epp) && (epp < ep) && (d > 0.0) && (dp > 0.0))) return 0.0;
+  const double cph = cos(ph);
+  const double bb = (1.0 + d * d) / (2.0 * ep) - (1.0 + dp * dp) / (2.0 * epp);
+  const double qsq = m * m * ((d * d + dp * dp - 2.0 * d * dp * cph) + m * m * bb * bb);
+  const double a = kAlpha / m;
+  const double PF = 8.0 / kPi * kAlpha * (a * a) * (epp * (m * m * m * m)) / (w * ep * (qsq * qsq)) * d * dp;
+  const double jac = kPi * ep * ep * (ep - m - egmin) / (m * m);
+  const double a0 = T.brem_a0;
+  const double Z = T.d->Z;
+  const double a0t = 1.0 + a0 * a0 * qsq;
+  const double FF = Z * Z * (a0 * a0 * a0 * a0) * (qsq * qsq) / (a0t * a0t);
+  const double A = 1.0 + d * d, B = 1.0 + dp * dp;
+  const double T1 = d * d / (A * A);
+  const double T2 = dp * dp / (B * B);
+  const double T3 = w * w / (2.0 * ep * epp) * (d * d + dp * dp) / (A * B);
+  const double T4 = -(epp / ep + ep / epp) * (d * dp * cph) / (A * B);
+  return PF * (T1 + T2 + T3 + T4) * jac * FF;
+}
+
+__device__ __noinline__ double f_pairprod_ref(const DevTables& T, double w, double x0, double x1, double x2,
+                                              double x3) {
+  const double x[4] = {x0, x1, x2, x3};
+  const double m = kMe;
+  const double epp = m + x[0] * (w - 2.0 * m);
+  const double dp = w / (2.0 * m) * (x[1] + x[2]);
+  const double dm = w / (2.0 * m) * (x[1] - x[2]);
+  const double ph = x[3] * 2.0 * kPi;
+  const double epm = w - epp;
+  if (!((m < epm) && (epm < w) && (m < epp) && (epp < w) && (dm > 0.0) && (dp > 0.0))) return 0.0;
+  const double cph = cos(ph);
+  const double S = (1.0 + dp * dp) / (2.0 * epp) + (1.0 + dm * dm) / (2.0 * epm);
+  const double qq = (dp * dp + dm * dm + 2.0 * dp * dm * cph) + m * m * S * S;
+  const double a = kAlpha / m;
+  const double PF = 8.0 / kPi * kAlpha * (a * a) * epp * epm / (w * w * w * (qq * qq)) * dp * dm;
+  const double jac = kPi * w * w * (w - 2.0 * m) / (m * m);
+  const double a0 = T.brem_a0;
+  const double Z = T.d->Z;
+  const double t = m * m * qq;
+  const double a0t = 1.0 + a0 * a0 * t;
+  const double FF = Z * Z * (a0 * a0 * a0 * a0) * (t * t) / (a0t * a0t);
+  const double A = 1.0 + dp * dp, B = 1.0 + dm * dm;
+  const double T1 = -1.0 * dp * dp / (A * A);
+  const double T2 = -1.0 * dm * dm / (B * B);
+  const double T3 = w * w / (2.0 * epp * epm) * (dp * dp + dm * dm) / (A * B);
+  const double T4 = (epp / epm + epm / epp) * (dp * dm * cph) / (A * B);
+  return PF * (T1 + T2 + T3 + T4) * jac * FF;
 }
 
 // Compton / DarkComp on a free electron at rest, all_processes.py:624-742.

@@ -16,7 +16,9 @@ import numpy as np
 import torch
 
 from .batch import ParticleBatch
-from .constants import (TWO_BODY_CODE, PID_DARK_VECTOR, all_process_code, alpha_em, dark_process_codes, mass_dict,
+from . import _lib
+from .constants import (TWO_BODY_CODE, PID_DARK_VECTOR, all_process_code, alpha_em, dark_process_codes,
+                        dimensionalities_dark, mass_dict,
                         meson_twobody_branchingratios)
 from .dark_tables import DarkTables
 from .device import DeviceContext
@@ -120,7 +122,12 @@ class DarkShower(Shower):
 
     def __init__(self, dict_dir, target_material, min_energy, mV_in_GeV, mode="exact", maxF_fudge_global=1,
                  max_n_integrators=int(1e4), kinetic_mixing=1.0, Zeff=29.508, bound_electron=True, g_e=None,
-                 active_processes=None, fast_MCS_mode=True, rescale_MCS=1, seed=None, below="clamp"):
+                 active_processes=None, fast_MCS_mode=True, rescale_MCS=1, seed=None, below="clamp",
+                 masses="shipped"):
+        """The reference's constructor (dark_shower.py:69-84).  Extra: below
+        ("clamp" or the reference's "wrap") for m_V under the lightest mass,
+        and masses: "shipped" picks among the reference's six trained masses,
+        "all" also among the masses made by petite_amd.tablegen.dark_masses."""
         super().__init__(dict_dir, target_material, min_energy, maxF_fudge_global=maxF_fudge_global,
                          max_n_integrators=max_n_integrators, fast_MCS_mode=fast_MCS_mode, seed=seed,
                          rescale_MCS=rescale_MCS)
@@ -132,7 +139,7 @@ class DarkShower(Shower):
         self.Zeff = Zeff
         self.g_e = g_e if g_e is not None else kinetic_mixing * np.sqrt(4 * np.pi * alpha_em)
         self._dark = DarkTables(self._tables, mV_in_GeV, mode=mode, Zeff=Zeff, bound_electron=bound_electron,
-                                kinetic_mixing=kinetic_mixing, below=below)
+                                kinetic_mixing=kinetic_mixing, below=below, masses=masses)
         self._dark.eps2 = self.g_e ** 2 / (4 * np.pi * alpha_em)
         self._mV = self._dark.mV
         self._mV_estimator = self._dark.mV_est
@@ -182,9 +189,25 @@ class DarkShower(Shower):
             ctx = DeviceContext(self._tables, self.min_energy, maxF_fudge=self._maxF_fudge_global,
                                 max_trials=int(self._max_n_integrators) * self._neval, MS=MS,
                                 rescale_MCS=self._MCS_rescale_factor, dark=self._dark)
+            self._train_flat_maps(ctx)
             ctx.dark_setup(self._device_active())
             self._ctx[key] = ctx
         return self._ctx[key]
+
+    def _train_flat_maps(self, ctx):
+        """Adapt on the GPU the maps of a generated mass (DarkTables starts
+        them flat on the generator's domains), with the reference's iteration
+        counts (all_processes.py:1085-1096), once per DarkShower: the trained
+        nodes replace the flat ones for every later context.  The float64
+        atomics of the accumulation make the nodes reproducible to rounding,
+        not bit for bit, between runs."""
+        from .dark_tables import MAP_ITERATIONS
+        src = self._dark.map_source
+        for P in [P for P, how in src.items() if how == "untrained"]:
+            nodes = ctx.train_map(P, n_iter=MAP_ITERATIONS[P])
+            E, _, cnt = self._dark.grids[P]
+            self._dark.grids[P] = (E, nodes.ravel(), cnt)
+            src[P] = "trained"
 
     def dark_yield(self, record, mV, eps2=1.0):
         """eps^2 * Y(m_V) of a shower record, differentiable in m_V and eps^2
@@ -214,6 +237,80 @@ class DarkShower(Shower):
         if process not in ("DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"):
             return 0.0
         return float(self._dark.bsm_weight(np.array([PID]), np.array([float(energy_initial)]), process)[0])
+
+    # ------------------------------------------------------------ sampling --
+    def draw_dark_sample(self, Einc, LU_Key=-1, process="DarkBrem", VB=False, sid=None):
+        """One accepted point of a dark process's accept/reject draw at energy
+        Einc (dark_shower.py:649-704), or None when no sample is found within
+        max_n_integrators * neval candidates.  The map is the one at LU_Key =
+        argmin|E_i - Einc| + 1 (LU_Key is ignored, as draw_sample's is).  With
+        VB the candidate count is appended.  Random numbers from Philox stream
+        `sid` (default: the next call counter of this DarkShower)."""
+        if process not in ("DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"):
+            raise Exception("Your process is not in the list")
+        s = self._seed_counter() if sid is None else int(sid)
+        ctx = self.device_context()
+        x, trials, w, rc = ctx.draw(np.array([all_process_code[process]], dtype=np.int32),
+                                    np.array([float(Einc)]), np.array([s], dtype=np.uint64), self._seed)
+        t = int(trials[0])
+        if rc == _lib.PETITE_ERR_NO_SAMPLE or t <= 0:
+            print("No Sample Found", process, Einc)
+            return None
+        xs = x[0, : dimensionalities_dark[process]].double().cpu().numpy()
+        return np.concatenate([xs, [t]]) if VB else xs
+
+    def draw_pe_sample(self, sid=None):
+        """The bound electron's momentum [GeV] from its 1s momentum density,
+        effective charge Zeff (dark_shower.py:706-719), on the GPU."""
+        from .device import draw_pe
+        s = self._seed_counter() if sid is None else int(sid)
+        pe = float(draw_pe([s], self._seed, self.Zeff)[0])
+        if pe < 0.0:
+            raise RuntimeError("draw_pe_sample: no sample in 65536 accept/reject pairs")
+        return pe
+
+    def produce_bsm_particle(self, p_original, process, weight=None, VB=False, sid=None):
+        """The dark vector an SM particle makes by `process`
+        (dark_shower.py:721-804): interaction energy from the drate tables,
+        scattering over the distance and energy loss, the threshold shortcuts,
+        draw_dark_sample and the process's kinematics, on the GPU
+        (petite_dark_reweight of a one-particle record).  Returns a Particle,
+        or None where the reference returns None (no sample, a drate row that
+        sums to zero).  The vector's weight is weight (default GetBSMWeights)
+        times the particle's weight; vectors are produced where GetBSMWeights
+        is positive (where generate_dark_shower asks for them), otherwise
+        ValueError.  The pair's random numbers come from the Philox stream
+        derived from `sid` (default: the next call counter) and the process."""
+        if process not in ("DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"):
+            raise Exception("Your process is not in the list")
+        if process not in self.active_processes:
+            raise ValueError(f"produce_bsm_particle: {process} is not among this DarkShower's active processes")
+        wg_dev = self.GetBSMWeights(p_original, process)
+        if not wg_dev > 0.0:
+            raise ValueError(f"produce_bsm_particle: GetBSMWeights is 0 for this particle and {process}; vectors "
+                             "are produced where it is positive")
+        store = self._store_from_particles([p_original])
+        store["sid"][0] = int(np.uint64(self._seed_counter() if sid is None else int(sid)).view(np.int64))
+        ctx = self.device_context()
+        out, n_e, stats = ctx.dark_reweight(store, 1, self._seed)
+        code = all_process_code[process]
+        meta = out["meta"][:n_e].cpu().numpy()
+        hit = np.nonzero((meta & 0xFF) == code)[0]
+        if len(hit) == 0 or not (meta[hit[0]] & 0x100):
+            return None
+        e = int(hit[0])
+        p = out["p"][e].double().cpu().numpy()
+        w = float(out["r"][e, 3])
+        if weight is not None:
+            w = w / wg_dev * float(weight) if wg_dev > 0 else 0.0
+        ids = p_original.get_ids()
+        name = process + ("_bound" if self.bound_electron and process in ("DarkAnn", "DarkComp") else "")
+        d = {"PID": PID_DARK_VECTOR, "parent_PID": ids["PID"], "ID": 2 * ids["ID"], "parent_ID": ids["ID"],
+             "generation_number": ids["generation_number"] + 1, "generation_process": name, "weight": w,
+             "mass": round(float(self._mV), 6)}
+        if VB:
+            print(f"{process}: vector from {p_original.get_ids()['PID']} at E0 = {p_original.get_p0()[0]}")
+        return Particle(p, np.asarray(p_original.get_rf(), dtype=np.float64), d)
 
     # ------------------------------------------------------------- records --
     @staticmethod

@@ -19,14 +19,97 @@ every table node; caches shipped with the reference (dark_weights.pkl,
 dark_drate.pkl) are used when they cover the (mass, material) pair, as the
 reference does.
 """
+import json
+import os
+
 import numpy as np
 
 from .constants import (TWO_BODY_CODE, GeVsqcm2, alpha_em, cmtom, m_electron, m_muon,
                         meson_twobody_branchingratios)
-from .tables import MaterialTables, _load, geometric
+from .tables import DATA_DIR, MaterialTables, _load, geometric, trained_grid, trained_pack
 
 DARK_PROCESSES = ["DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"]
 _GL_X, _GL_W = np.polynomial.legendre.leggauss(8)
+
+
+class _Packs:
+    """Several npz packs read as one: ``files`` is their union and an item
+    comes from the first pack that holds it."""
+
+    def __init__(self, packs):
+        self._packs = list(packs)
+        self.files = [f for z in self._packs for f in z.files]
+
+    def __getitem__(self, key):
+        for z in self._packs:
+            if key in z.files:
+                return z[key]
+        raise KeyError(key)
+
+
+def generator_energies(process, mV, E_max=100.0, n=100):
+    """The energies at which the reference's generator tabulates a dark
+    process at mass mV (utilities/script_generateintegrators.py:62-83)."""
+    if process == "DarkBrem":
+        return np.geomspace(1.25 * mV, E_max, n)
+    if process == "DarkMuonBrem":
+        return np.geomspace(1.25 * (mV + m_muon), E_max, n)
+    if process == "DarkAnn":
+        E0 = (mV ** 2 - 2 * m_electron ** 2) / (2 * m_electron)
+    else:
+        E0 = mV * (1 + mV / (2 * m_electron))
+    E_hi = max(E_max, 100 * E0)
+    return E0 * (1 + np.geomspace(1e-4, (E_hi - E0) / E0, n))
+
+
+MAP_INCREMENTS = 1000
+MAP_ITERATIONS = {"DarkBrem": 20, "DarkMuonBrem": 20, "DarkAnn": 10, "DarkComp": 10}  # all_processes.py:1085-1096
+
+
+def flat_map(process, E, mV):
+    """Uniform maps on the generator's domains (all_processes.py:1102-1158) at
+    the energies E, 1000 increments per dimension as the shipped maps:
+    DarkBrem / DarkMuonBrem x in [m_V/E, 1 - m_e/E], log10(1 - cos theta) in
+    [-12, log10 2], log10(t~) in [-20, 0]; DarkAnn u in [0, 1]; DarkComp
+    cos theta in [-1, 1].  Returns (E, nodes float32, node counts)."""
+    rows = []
+    for e in E:
+        if process in ("DarkBrem", "DarkMuonBrem"):
+            dom = [(max(0.0, mV / e), 1.0 - m_electron / e), (-12.0, np.log10(2.0)), (-20.0, 0.0)]
+        elif process == "DarkAnn":
+            dom = [(0.0, 1.0)]
+        else:
+            dom = [(-1.0, 1.0)]
+        rows.append(np.concatenate([np.linspace(a, b, MAP_INCREMENTS + 1) for a, b in dom]))
+    ndim = 3 if process in ("DarkBrem", "DarkMuonBrem") else 1
+    return (np.asarray(E, dtype=np.float64), np.asarray(rows, dtype=np.float32).ravel(),
+            np.full((len(E), ndim), MAP_INCREMENTS + 1))
+
+
+def mass_pack(table_set):
+    """The pack of dark tables at the masses made by
+    petite_amd.tablegen.dark_masses (dark cross sections per material and
+    maps at masses the reference ships none for), or None."""
+    path = os.path.join(DATA_DIR, f"dark_{table_set}_masses.npz")
+    if not os.path.exists(path):
+        return None
+    z = np.load(path)
+    return z, json.loads(bytes(z["meta_json"]).decode())
+
+
+def dark_pack(table_set, masses="shipped"):
+    """(pack, trained masses) of a table set: the shipped dark tables, and with
+    masses="all" also those of the generated mass pack."""
+    z, meta = _load("dark", table_set)
+    mv = [float(m) for m in meta["masses"]]
+    if masses == "all":
+        extra = mass_pack(table_set)
+        if extra is not None:
+            mv = sorted(set(mv) | {float(m) for m in extra[1]["masses"]})
+            return _Packs([z, extra[0]]), mv
+    elif masses != "shipped":
+        raise ValueError('masses: "shipped" or "all"')
+    return z, mv
 
 
 def closest_lesser_value(values, x, below="clamp"):
@@ -172,14 +255,13 @@ class DarkTables:
     """Dark tables of one material, table set and mass."""
 
     def __init__(self, sm: MaterialTables, mV, mode="exact", Zeff=29.508, bound_electron=True, kinetic_mixing=1.0,
-                 below="clamp"):
+                 below="clamp", masses="shipped"):
         if not bound_electron:
             raise NotImplementedError("only bound_electron=True (the reference's default) is implemented")
-        z, meta = _load("dark", sm.table_set)
-        self.meta = meta
+        z, self.mV_list = dark_pack(sm.table_set, masses)
+        shipped = [float(m) for m in _load("dark", sm.table_set)[1]["masses"]]
         self.sm = sm
         self.material = sm.material
-        self.mV_list = list(meta["masses"])
         if mode == "exact":
             self.mV = closest_lesser_value(self.mV_list, mV, below)
             self.mV_est = self.mV
@@ -193,6 +275,7 @@ class DarkTables:
         self.eps2 = kinetic_mixing ** 2  # g_e^2 / (4 pi alpha) (dark_shower.py:109-111)
         key = repr(self.mV_est)
         mat = self.material
+        self._generated = float(self.mV_est) not in shipped
 
         def xs(P):
             return np.array(z[f"xsec__{P}__{key}__{mat}"], dtype=np.float64)
@@ -224,21 +307,36 @@ class DarkTables:
         self._build_survival()
         self._build_weights(z, key)
         self.grids = {}
+        self.map_source = {}
         fallback = None
+        tp = trained_pack(sm.table_set) if sm.table_set != "100GeV" else None
         for P in DARK_PROCESSES:
             gk = f"{P}__{key}"
-            src = z
-            if f"E__{gk}" not in z.files and sm.table_set != "100GeV":
-                # the 400 GeV set ships no DarkBrem/DarkMuonBrem maps: use the
-                # 100 GeV maps of the same mass (their variables are
-                # dimensionless; energies above the last grid energy take the
-                # last map, as LU_Key clamps)
+            if f"E__{gk}" in z.files:
+                self.grids[P] = (np.array(z[f"E__{gk}"]), np.array(z[f"nodes__{gk}"], dtype=np.float32),
+                                 np.array(z[f"nodecnt__{gk}"]))
+                self.map_source[P] = "shipped"
+            elif self._generated:
+                # a generated mass: its pack holds cross sections only; the
+                # maps start flat on the generator's domains at the table
+                # energies and DarkShower trains them on the GPU
+                self.grids[P] = flat_map(P, np.array(z[f"xsec__{P}__{key}__{mat}"])[:, 0], self.mV_est)
+                self.map_source[P] = "untrained"
+            elif sm.table_set != "100GeV":
+                # the 400 GeV set ships no DarkBrem/DarkMuonBrem maps: the
+                # pack of GPU-trained maps when it has them, else the 100 GeV
+                # maps of the same mass (their variables are dimensionless;
+                # energies above the last grid energy take the last map, as
+                # LU_Key clamps)
                 if fallback is None:
                     fallback, _ = _load("dark", "100GeV")
-                src = fallback
-            if f"E__{gk}" in src.files:
-                self.grids[P] = (np.array(src[f"E__{gk}"]), np.array(src[f"nodes__{gk}"], dtype=np.float32),
-                                 np.array(src[f"nodecnt__{gk}"]))
+                if f"E__{gk}" not in fallback.files:
+                    continue
+                standin = (np.array(fallback[f"E__{gk}"]), np.array(fallback[f"nodes__{gk}"], dtype=np.float32),
+                           np.array(fallback[f"nodecnt__{gk}"]))
+                g = trained_grid(tp, gk, standin) if tp is not None else None
+                self.grids[P] = g if g is not None else standin
+                self.map_source[P] = "trained" if g is not None else "100GeV stand-in"
         # n*sigma of the dark processes for the device (log-log -> tabulate as given)
         self.nsigma_tab = {}
 

@@ -7,7 +7,10 @@ Y(m_V) = sum_i w_i sum_P W_P(E0_i; m_V) with W_P = GetBSMWeights (595-647),
 interpolates each particle's weight between the trained masses m_j < m_{j+1}:
 W_j^(1-t) W_{j+1}^t (t = ln(m_V/m_j) / ln(m_{j+1}/m_j)) where both are positive,
 (1-t) W_j + t W_{j+1} otherwise; at a trained mass it equals the weight sum of
-DarkShower.generate_dark_shower's entries.  One float64 pass of the
+DarkShower.generate_dark_shower's entries.  The trained masses are the
+shipped ones and those of the generated mass pack
+(petite_amd.tablegen.dark_masses); a mass outside [lightest, heaviest] raises
+PetiteError (petite_dark_yield rejects it: nothing to interpolate).  One float64 pass of the
 k_dark_yield kernel returns Y and dY/dm_V for up to 16 masses at a time;
 dY/d(eps^2) = Y.  ``DarkYield.__call__`` wraps it in a torch.autograd.Function.
 """
@@ -37,14 +40,14 @@ class DarkYield:
         active = list(dark_process_codes if active_processes is None else active_processes)
         self.active_codes = [all_process_code[p] for p in active if p in all_process_code]
         self.dark = {}
-        probe = DarkTables(tables, 1.0, Zeff=Zeff)
+        probe = DarkTables(tables, 1.0, Zeff=Zeff, masses="all")
         self.masses = sorted(float(m) for m in (probe.mV_list if masses is None else masses))
         if len(self.masses) > _lib.MAX_MASSES:
             raise ValueError(f"at most {_lib.MAX_MASSES} trained masses")
         descs = (_lib.DarkDesc * len(self.masses))()
         parts, n = [], 0
         for j, m in enumerate(self.masses):
-            dt = probe if m == probe.mV else DarkTables(tables, m, Zeff=Zeff)
+            dt = probe if m == probe.mV else DarkTables(tables, m, Zeff=Zeff, masses="all")
             if dt.mV != m:
                 raise ValueError(f"mass {m} is not a trained mass ({dt.mV_list})")
             dt.eps2 = 1.0
@@ -60,15 +63,23 @@ class DarkYield:
         self._blob = np.concatenate(parts)
         self._descs = descs
         self.ctx = ctx if ctx is not None else DeviceContext(tables, min_energy)
+        self._setup()
+
+    def _setup(self):
+        """Load this model's tables into the context (a context holds one
+        model's tables; models sharing a context reload theirs when used)."""
         m = np.ascontiguousarray(self.masses, dtype=np.float64)
         _lib.check(self.ctx.lib.petite_yield_setup(self.ctx.ctx, len(self.masses), m.ctypes.data,
-                                                   C.cast(descs, C.c_void_p), self._blob.ctypes.data,
+                                                   C.cast(self._descs, C.c_void_p), self._blob.ctypes.data,
                                                    self._blob.size), "petite_yield_setup")
+        self.ctx._yield_owner = self
 
     def evaluate(self, record, mV, grad=True):
         """(Y, dY/dm_V) per unit eps^2 at the masses mV (host float64 arrays);
         grad=False skips the derivative (returns None for it)."""
         store, n = (record._store, record.n) if hasattr(record, "_store") else record
+        if getattr(self.ctx, "_yield_owner", None) is not self:
+            self._setup()
         mv = np.ascontiguousarray(np.atleast_1d(mV), dtype=np.float64)
         Y = np.zeros_like(mv)
         dY = np.zeros_like(mv) if grad else None

@@ -4,6 +4,7 @@ Device memory for inputs and outputs is allocated with PyTorch (ROCm) and
 handed to the C ABI as raw pointers; the context holds the table copies.
 """
 import ctypes as C
+import os
 
 import numpy as np
 import torch
@@ -14,6 +15,34 @@ from .tables import MaterialTables, build_blobs
 
 MAXF_SEED = 0x5EEDF00D
 MAXF_SAMPLES = 32768  # ~ n_trials (100) x neval (300) of utilities/find_maxes.py:206-210
+
+
+VERTEX_DTYPE = np.dtype([("E", "f8"), ("px", "f8"), ("py", "f8"), ("pz", "f8"), ("x", "f8"), ("y", "f8"),
+                         ("z", "f8"), ("weight", "f8"), ("pid", "i4"), ("generation", "i4"), ("sid", "u8")])
+STORE_KEYS = ("p0", "r0", "pf", "rf", "pid", "parent", "meta", "shower", "sid")
+
+
+def _empty_store(capacity, dev):
+    return {k: torch.zeros((capacity, 4) if k in ("p0", "r0", "pf", "rf") else (capacity,), device=dev,
+                           dtype=torch.float32 if k in ("p0", "r0", "pf", "rf") else
+                           (torch.int64 if k == "sid" else torch.int32)) for k in STORE_KEYS}
+
+
+def _store_struct(store):
+    return _lib.Store(*(store[k].data_ptr() for k in STORE_KEYS), int(store["pid"].shape[0]))
+
+
+def draw_pe(sid, seed, Zeff):
+    """petite_draw_pe: bound-electron momenta [GeV] for the given stream ids
+    (-1 where no sample came within 65536 pairs)."""
+    _require_gpu()
+    lib = _lib.load()
+    dev = torch.device("cuda")
+    s_t = torch.as_tensor(np.asarray(sid, dtype=np.uint64).view(np.int64), device=dev)
+    out = torch.empty(s_t.numel(), device=dev, dtype=torch.float64)
+    _lib.check(lib.petite_draw_pe(s_t.data_ptr(), s_t.numel(), int(seed), float(Zeff), out.data_ptr(),
+                                  _stream_handle()), "petite_draw_pe")
+    return out
 
 
 def _stream_handle():
@@ -27,7 +56,11 @@ def _require_gpu():
 
 class DeviceContext:
     def __init__(self, tables: MaterialTables, min_energy, maxF_fudge=1.0, max_trials=int(1e4) * 300, MS=True,
-                 rescale_MCS=1.0, dark=None, maxf_samples=MAXF_SAMPLES):
+                 rescale_MCS=1.0, dark=None, maxf_samples=MAXF_SAMPLES, mT=None):
+        """mT: the target mass the integrands see (default A_T, the event_info
+        of draw_sample, shower.py:435); the cross-section tables were made
+        with the nuclear mass (utilities/find_maxes.py:88), which a context
+        that regenerates them passes here."""
         _require_gpu()
         self.lib = _lib.load()
         self.tables = tables
@@ -38,6 +71,8 @@ class DeviceContext:
         desc.maxF_fudge = float(maxF_fudge)
         desc.max_trials = int(max_trials)
         desc.MS = 1 if MS else 0
+        if mT is not None:
+            desc.mT = float(mT)
         if dark is not None:
             desc.mV = float(dark.mV)
             desc.Zeff = float(dark.Zeff)
@@ -47,6 +82,15 @@ class DeviceContext:
         _lib.check(self.lib.petite_ctx_create(C.byref(desc), b64.ctypes.data, b64.size, b32.ctypes.data, b32.size,
                                               C.byref(ctx)), "petite_ctx_create")
         self.ctx = ctx
+        # launch sizes tuned on this kind of device (petite_amd.autotune), unless
+        # PETITE_TUNING=default asks for the built-in ones
+        self.tuning_source = "default"
+        if os.environ.get("PETITE_TUNING", "") != "default":
+            from .autotune import load
+            pt = load()
+            if pt is not None:
+                self.set_tuning(pt)
+                self.tuning_source = "table"
         self.maxf = {}
         self.sigma_est = {}
         for P, code in all_process_code.items():
@@ -66,6 +110,22 @@ class DeviceContext:
 
     def set_timing(self, on=True):
         _lib.check(self.lib.petite_set_timing(self.ctx, 1 if on else 0), "petite_set_timing")
+
+    def set_tuning(self, tuning=None):
+        """Launch and queue sizes of run_showers (dict of PetiteTuning fields;
+        missing or non-positive fields keep their defaults; None restores all
+        defaults).  Returns the values in effect."""
+        if tuning is None:
+            _lib.check(self.lib.petite_set_tuning(self.ctx, None), "petite_set_tuning")
+        else:
+            t = _lib.Tuning.from_dict(tuning) if isinstance(tuning, dict) else tuning
+            _lib.check(self.lib.petite_set_tuning(self.ctx, C.byref(t)), "petite_set_tuning")
+        return self.get_tuning()
+
+    def get_tuning(self):
+        t = _lib.Tuning()
+        _lib.check(self.lib.petite_get_tuning(self.ctx, C.byref(t)), "petite_get_tuning")
+        return t.as_dict()
 
     def scan_maxf(self, process, n_samples=MAXF_SAMPLES, seed=MAXF_SEED):
         code = all_process_code[process]
@@ -124,11 +184,38 @@ class DeviceContext:
         x = torch.empty((n, 4), device=dev, dtype=torch.float32)
         trials = torch.empty(n, device=dev, dtype=torch.int32)
         w = torch.empty(n, device=dev, dtype=torch.float32)
-        rc = self.lib.petite_vegas_draw(self.ctx, proc_t.data_ptr(), E_t.data_ptr(), sid_t.data_ptr(), n, int(seed),
-                                        x.data_ptr(), trials.data_ptr(), w.data_ptr(), _stream_handle())
+        rc = self.lib.petite_vegas_draw(self.ctx, proc_t.data_ptr(), E_t.data_ptr(), sid_t.data_ptr(), n, int(seed), x.data_ptr(),
+                trials.data_ptr(), w.data_ptr(), _stream_handle())
         if rc not in (_lib.PETITE_OK, _lib.PETITE_ERR_NO_SAMPLE):
             _lib.check(rc, "petite_vegas_draw")
         return x, trials, w, rc
+
+    def sample_scattering(self, vertices, proc, seed):
+        """petite_sample_scattering: vertices is a structured array of
+        VERTEX_DTYPE, proc the SM process codes.  Returns (store dict of 2n
+        child entries, trials (n,), rc)."""
+        v = np.ascontiguousarray(vertices, dtype=VERTEX_DTYPE)
+        n = len(v)
+        dev = torch.device("cuda")
+        v_t = torch.from_numpy(v.view(np.uint8).copy()).to(dev)
+        p_t = torch.as_tensor(np.asarray(proc, dtype=np.int32), device=dev)
+        if p_t.numel() != n:
+            raise ValueError("sample_scattering: one process per vertex")
+        store = _empty_store(max(2 * n, 1), dev)
+        S = _store_struct(store)
+        trials = torch.zeros(max(n, 1), device=dev, dtype=torch.int32)
+        rc = self.lib.petite_sample_scattering(self.ctx, v_t.data_ptr(), p_t.data_ptr(), n, int(seed), C.byref(S),
+                                               trials.data_ptr(), _stream_handle())
+        if rc not in (_lib.PETITE_OK, _lib.PETITE_ERR_NO_SAMPLE):
+            _lib.check(rc, "petite_sample_scattering")
+        return store, trials[:n], rc
+
+    def propagate(self, store, n, seed):
+        """petite_propagate over the first n records of a store dict (p0, r0,
+        pid, meta, shower, sid filled in); pf and rf are written in place."""
+        S = _store_struct(store)
+        _lib.check(self.lib.petite_propagate(self.ctx, C.byref(S), int(n), int(seed), _stream_handle()),
+                   "petite_propagate")
 
     def eval_points(self, proc, E, words):
         """J*f and x of single candidates given their Philox words (n x 4 uint32)."""

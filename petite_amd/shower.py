@@ -276,6 +276,97 @@ class Shower:
         self._n_calls += 1
         return (1 << 63) | c
 
+    def _particle_min_energy(self, p):
+        ids = p.get_ids()
+        return max(self._minimum_calculable_energy.get(ids["PID"], 0.0), self.min_energy, ids["mass"] or 0.0)
+
+    def sample_scattering(self, p0, process, VB=False, sid=None):
+        """The two particles of a hard interaction of p0 by `process`
+        (shower.py:467-507): draw_sample at p0's final energy, the process's
+        kinematics rotated onto p0's final direction, children at p0's final
+        position with the reference's ids.  None at or below the particle's
+        minimum energy.  The draw, the kinematic azimuth and the children's
+        stream ids come from Philox stream `sid` (default: the next call
+        counter of this Shower), on the GPU (petite_sample_scattering)."""
+        E0 = float(p0.get_pf()[0])
+        ids = p0.get_ids()
+        if E0 <= self._particle_min_energy(p0):
+            return None
+        if process not in process_code:
+            raise Exception("Your process is not in the list")
+        code = process_code[process]
+        from .device import VERTEX_DTYPE
+        v = np.zeros(1, dtype=VERTEX_DTYPE)
+        pf = np.asarray(p0.get_pf(), dtype=np.float64)
+        rf = np.asarray(p0.get_rf(), dtype=np.float64)
+        v["E"], v["px"], v["py"], v["pz"] = pf
+        v["x"], v["y"], v["z"] = rf
+        v["weight"] = float(ids["weight"])
+        v["pid"] = int(ids["PID"])
+        v["generation"] = int(ids["generation_number"])
+        v["sid"] = self._seed_counter() if sid is None else int(sid)
+        ctx = self.device_context()
+        store, trials, rc = ctx.sample_scattering(v, [code], self._seed)
+        t = int(trials[0])
+        if rc == _lib.PETITE_ERR_NO_SAMPLE or t <= 0:
+            raise Exception("No Sample Found", process, E0)
+        if VB:
+            print(f"{process}: sample found after {t} candidates")
+        p = store["p0"][:2].double().cpu().numpy()
+        pid = store["pid"][:2].cpu().numpy()
+        out = []
+        for k in range(2):
+            d = {"PID": int(pid[k]), "parent_PID": ids["PID"], "ID": 2 * ids["ID"] + k, "parent_ID": ids["ID"],
+                 "generation_number": ids["generation_number"] + 1, "generation_process": process,
+                 "weight": ids["weight"], "mass": mass_dict[int(pid[k])]}
+            out.append(Particle(p[k], rf, d))
+        return out
+
+    def propagate_particle(self, Part0, Losses=False, MS=False, sid=None):
+        """Transport to the next hard interaction (shower.py:509-601), as
+        generate_shower calls it: photons with Losses=False and MS=False (one
+        exponential flight), charged leptons with Losses = the material's
+        dE/dx [GeV/m] and MS = the multiple-scattering flag (substeps with
+        energy loss, and scattering when MS).  Other combinations are not
+        supported on the device (NotImplementedError).  Random numbers from
+        Philox stream `sid` (default: the next call counter of this Shower)."""
+        if Part0.get_ended() is True:
+            Part0.set_rf(Part0.get_rf())
+            return Part0
+        if (np.linalg.norm(np.asarray(Part0.get_p0()) - np.asarray(Part0.get_pf())) > 0.0) \
+                or (np.linalg.norm(np.asarray(Part0.get_r0()) - np.asarray(Part0.get_rf())) > 0.0):
+            raise ValueError("propagate_particle() should only be called for a particle with pf = p0 and rf = r0 "
+                             "and get_ended() == False")
+        pid = int(Part0.get_ids()["PID"])
+        if Part0.get_p0()[0] < self._particle_min_energy(Part0):
+            Part0.set_ended(True)
+            return Part0
+        if pid == 22:
+            if Losses or MS:
+                raise NotImplementedError("propagate_particle: photons propagate with Losses=False, MS=False "
+                                          "(generate_shower's call) on the device")
+        elif pid in _CHARGED:
+            if not Losses or abs(float(Losses) - self._tables.dEdx) > 1e-12 * self._tables.dEdx:
+                raise NotImplementedError("propagate_particle: charged leptons propagate with Losses = the "
+                                          f"material's dE/dx ({self._tables.dEdx} GeV/m) on the device")
+        else:
+            raise NotImplementedError(f"propagate_particle: PID {pid} is not propagated by generate_shower")
+        from .device import _empty_store
+        dev = torch.device("cuda")
+        store = _empty_store(1, dev)
+        rec = self._primary_record(Part0, 0, self._seed_counter() if sid is None else int(sid))
+        store["p0"][0] = torch.tensor([rec["E"][0], rec["px"][0], rec["py"][0], rec["pz"][0]])
+        store["r0"][0] = torch.tensor([rec["x"][0], rec["y"][0], rec["z"][0], rec["weight"][0]])
+        store["pid"][0] = pid
+        store["meta"][0] = int(Part0.get_ids()["generation_number"]) & 0xFFFF
+        store["sid"][0] = int(np.asarray(rec["sid"][0]).view(np.int64))
+        ctx = self.device_context(MS=bool(MS) if pid != 22 else True)
+        ctx.propagate(store, 1, self._seed)
+        Part0.set_pf(store["pf"][0].double().cpu().numpy())
+        Part0.set_rf(store["rf"][0, :3].double().cpu().numpy())
+        Part0.set_ended(True)
+        return Part0
+
     # ------------------------------------------------------------- showers --
     @staticmethod
     def _primary_record(p0, shower, sid):

@@ -28,6 +28,47 @@ def _load(kind, table_set):
     return z, json.loads(bytes(z["meta_json"]).decode())
 
 
+def trained_pack(table_set):
+    """The pack of maps trained on the GPU for the maps a table set lacks
+    (petite_amd.tablegen.train_missing_maps), or None."""
+    path = os.path.join(DATA_DIR, f"maps_{table_set}_trained.npz")
+    if not os.path.exists(path):
+        return None
+    z = np.load(path)
+    return z, json.loads(bytes(z["meta_json"]).decode())
+
+
+def trained_grid(pack, key, standin):
+    """A map of the trained pack: its energies, the pack's nodes at the
+    trained energies and the stand-in map (E, nodes, counts) put on the other
+    energies (nearest in log E, as regrid does).  None when the pack lacks it."""
+    z = pack[0]
+    if f"E__{key}" not in z.files:
+        return None
+    E = np.array(z[f"E__{key}"], dtype=np.float64)
+    g = {"m": standin}
+    regrid(g, "m", E)
+    _, nodes, cnt = g["m"]
+    stride = int(np.asarray(cnt)[0].sum())
+    nodes = np.array(nodes, dtype=np.float32).reshape(len(E), stride)
+    idx = np.array(z[f"trained__{key}"], dtype=np.int64)
+    nodes[idx] = np.array(z[f"nodes__{key}"], dtype=np.float32).reshape(len(idx), stride)
+    return E, nodes.ravel(), np.array(z[f"nodecnt__{key}"])
+
+
+def grid_cuts(meta):
+    """(Eg_min, Ee_min) of a table set: the parameters of its Brem grid
+    (shower.py:214-215), else of the first grid that records them (the 400 GeV
+    set ships no Brem grid; all its grids carry the same generator
+    parameters)."""
+    if meta.get("Eg_min") is not None and meta.get("Ee_min") is not None:
+        return float(meta["Eg_min"]), float(meta["Ee_min"])
+    for g in meta["grids"].values():
+        if g and g.get("Eg_min") is not None and g.get("Ee_min") is not None:
+            return float(g["Eg_min"]), float(g["Ee_min"])
+    raise ValueError("no grid of the table set records Eg_min / Ee_min")
+
+
 def table_set_for(dict_dir):
     """Map the reference's dict_dir argument to a table set name."""
     if dict_dir is None:
@@ -106,8 +147,7 @@ class MaterialTables:
         self.dEdx = ti["dEdx"] * 0.1  # GeV/m (shower.py:649)
         self.nZ = self.rho / m_proton_grams / self.A
         self.ne = self.nZ * self.Z
-        self.Eg_min = float(meta["Eg_min"] if meta["Eg_min"] is not None else 0.001)
-        self.Ee_min = float(meta["Ee_min"] if meta["Ee_min"] is not None else 0.005)
+        self.Eg_min, self.Ee_min = grid_cuts(meta)
         xs = {P: np.array(z[f"xsec__{P}__{target_material}"], dtype=np.float64) for P in SM_PROCESSES}
         mue = xs["MuonE"]
         while mue[0, 1] == 0.0:  # shower.py:238-239
@@ -138,18 +178,26 @@ class MaterialTables:
         me_E = np.geomspace(emin, bm[-1], len(BS))  # shower.py:289-293
         ns["MuonE"] = (me_E, self.ne * GeVsqcm2 * sigma_muone(me_E, self.Ee_min))
         self.nsigma_tab = ns
-        # grids (the 400 GeV set lacks the Brem/PairProd/MuonBrem maps: those
-        # come from the 100 GeV set; a map only changes sampling efficiency)
+        # grids.  The 400 GeV set lacks the Brem/PairProd/MuonBrem maps: they
+        # come from the pack of GPU-trained maps when it is there, else the
+        # 100 GeV maps stand in (a map only changes sampling efficiency)
         self.grids = {}
+        self.map_source = {}
         fb = None
+        tp = trained_pack(table_set)
         for P in SM_PROCESSES:
-            src = z
-            if f"E__{P}" not in z.files:
-                if fb is None:
-                    fb, _ = _load("sm", "100GeV")
-                src = fb
-            self.grids[P] = (np.array(src[f"E__{P}"]), np.array(src[f"nodes__{P}"], dtype=np.float32),
-                             np.array(src[f"nodecnt__{P}"]))
+            if f"E__{P}" in z.files:
+                self.grids[P] = (np.array(z[f"E__{P}"]), np.array(z[f"nodes__{P}"], dtype=np.float32),
+                                 np.array(z[f"nodecnt__{P}"]))
+                self.map_source[P] = "shipped"
+                continue
+            if fb is None:
+                fb, _ = _load("sm", "100GeV")
+            standin = (np.array(fb[f"E__{P}"]), np.array(fb[f"nodes__{P}"], dtype=np.float32),
+                       np.array(fb[f"nodecnt__{P}"]))
+            g = trained_grid(tp, P, standin) if tp is not None else None
+            self.grids[P] = g if g is not None else standin
+            self.map_source[P] = "trained" if g is not None else "100GeV stand-in"
 
     def regrid_map(self, process, E_new):
         regrid(self.grids, process, E_new)

@@ -67,9 +67,10 @@ def refine_nodes(nodes, sum_f, n_f, alpha=0.5):
     return new
 
 
-def train_process(ctx, process, n_iter=10, n_per_energy=1 << 16, alpha=0.5, seed=TRAIN_SEED):
+def train_process(ctx, process, n_iter=10, n_per_energy=1 << 16, alpha=0.5, seed=TRAIN_SEED, train=None):
     """Train the context's grid of `process` in place (n_iter adaptations at
-    every grid energy) and return the new float32 nodes (n_E, stride)."""
+    every grid energy, or at the energy indices where the boolean mask
+    `train` is set) and return the new float32 nodes (n_E, stride)."""
     from .constants import all_process_code
     code = all_process_code[process]
     g = ctx.desc.grid[code]
@@ -78,6 +79,9 @@ def train_process(ctx, process, n_iter=10, n_per_energy=1 << 16, alpha=0.5, seed
         raise ValueError(f"train_process: {process} has no grid in this context")
     ninc = [int(g.ninc[d]) for d in range(ndim)]
     E, nodes32, cnt = ctx.tables_grid(process)
+    todo = np.ones(n_E, dtype=bool) if train is None else np.asarray(train, dtype=bool)
+    if todo.shape != (n_E,):
+        raise ValueError("train_process: the mask needs one entry per grid energy")
     nodes = np.asarray(nodes32, dtype=np.float32).reshape(n_E, stride).copy()
     dev = torch.device("cuda")
     sum_f = torch.zeros(n_E * stride, device=dev, dtype=torch.float64)
@@ -91,7 +95,7 @@ def train_process(ctx, process, n_iter=10, n_per_energy=1 << 16, alpha=0.5, seed
         S = sum_f.cpu().numpy().reshape(n_E, stride)
         Nn = n_f.cpu().numpy().reshape(n_E, stride)
         new = nodes.astype(np.float64)
-        for ie in range(n_E):
+        for ie in np.nonzero(todo)[0]:
             off = 0
             for d in range(ndim):
                 N = ninc[d]

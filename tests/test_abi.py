@@ -35,7 +35,8 @@ def test_struct_layouts_match_c(tmp_path):
                    'sizeof(PetitePrimary), sizeof(PetiteStore), sizeof(PetiteStats), offsetof(PetiteTableDesc, grid), '
                    'sizeof(PetiteGridDesc), sizeof(PetiteSigmaDesc), sizeof(PetiteDarkDesc), '
                    'offsetof(PetiteDarkDesc, eps2), sizeof(PetiteDarkOut), sizeof(PetiteDarkStats), '
-                   'offsetof(PetiteDarkStats, yield)); return 0;}\n')
+                   'offsetof(PetiteDarkStats, yield)); printf("%zu %zu\\n", sizeof(PetiteTuning), '
+                   'offsetof(PetiteTuning, sweep_bound)); return 0;}\n')
     exe = tmp_path / "sz"
     try:
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
@@ -45,7 +46,8 @@ def test_struct_layouts_match_c(tmp_path):
     want = [ctypes.sizeof(_lib.TableDesc), ctypes.sizeof(_lib.Primary), ctypes.sizeof(_lib.Store),
             ctypes.sizeof(_lib.Stats), _lib.TableDesc.grid.offset, ctypes.sizeof(_lib.GridDesc),
             ctypes.sizeof(_lib.SigmaDesc), ctypes.sizeof(_lib.DarkDesc), _lib.DarkDesc.eps2.offset,
-            ctypes.sizeof(_lib.DarkOut), ctypes.sizeof(_lib.DarkStats), _lib.DarkStats.yield_.offset]
+            ctypes.sizeof(_lib.DarkOut), ctypes.sizeof(_lib.DarkStats), _lib.DarkStats.yield_.offset,
+            ctypes.sizeof(_lib.Tuning), _lib.Tuning.sweep_bound.offset]
     assert got == want
 
 
@@ -74,3 +76,18 @@ def test_detector_limit_matches_header():
     from petite_amd import detector
     m = re.search(r"#define PETITE_MAX_DETECTORS (\d+)", open(HEADER).read())
     assert int(m.group(1)) == detector.MAX_DETECTORS
+
+
+def test_tuning_rejects_null_context_without_gpu():
+    lib = _lib.load()
+    t = _lib.Tuning()
+    assert lib.petite_set_tuning(None, ctypes.byref(t)) == _lib.PETITE_ERR_ARG
+    assert lib.petite_get_tuning(None, ctypes.byref(t)) == _lib.PETITE_ERR_ARG
+    assert b"petite_get_tuning" in lib.petite_last_error()
+
+
+def test_tuning_dict_round_trip():
+    d = {"batch_lanes": 40, "yield_steps": 32, "transport_waves": 16, "emit_waves": 24,
+         "draw_waves": [12, 2, 8, 2, 2, 2, 2, 2], "min_group_lanes": 4, "sweep_group": 8, "group_requests": 2,
+         "sweep_bound": 1 << 22, "resume_cap": 1 << 19}
+    assert _lib.Tuning.from_dict(d).as_dict() == d

@@ -1,9 +1,10 @@
 """GPU parity of the accept/reject draw and the max_F scan against the oracle.
 
 Identical Philox streams on both sides: the accepted candidate index (trial
-count) must agree except where float32 rounding of J*f moves a candidate
-across the acceptance threshold, and the accepted points must agree to float32
-rounding of the map transform.
+count) is index work and must agree in every draw.  The kernel's float32
+4-D integrands decide alone only outside a 4e-4 relative band around the
+threshold max_F * u; inside it the float64 reference formula decides, as the
+oracle does.  The accepted points agree to float32 rounding of the stored x.
 """
 import numpy as np
 import pytest
@@ -49,11 +50,12 @@ def _energies(T, P, n, rng):
     return np.exp(rng.uniform(np.log(lo * 1.02), np.log(hi * 0.999), n))
 
 
+@pytest.mark.parametrize("mat", ["graphite", "lead"])
 @pytest.mark.parametrize("P", SM)
-def test_draw_matches_oracle(ctxs, P):
-    ctx, T = ctxs["graphite"]
-    rng = np.random.default_rng(7)
-    n = 300 if P in ("Brem", "PairProd", "MuonBrem") else 500
+def test_draw_matches_oracle(ctxs, mat, P):
+    ctx, T = ctxs[mat]
+    rng = np.random.default_rng(7 if mat == "graphite" else 8)
+    n = 600 if P in ("Brem", "PairProd", "MuonBrem") else 1000
     E = _energies(T, P, n, rng)
     sid = np.arange(n, dtype=np.uint64) + 1000
     x, trials, w, rc = ctx.draw(np.full(n, process_code[P], np.int32), E, sid, SEED)
@@ -70,4 +72,4 @@ def test_draw_matches_oracle(ctxs, P):
             same += 1
             d = g.ndim
             assert np.allclose(x[i, :d], xo[:d], rtol=2e-6, atol=2e-6), (P, i, x[i, :d], xo)
-    assert same >= 0.98 * n, (P, same, n)
+    assert same == n, (P, mat, same, n)

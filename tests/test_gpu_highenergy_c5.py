@@ -1,0 +1,99 @@
+"""The 400 GeV workload: 10,000 primaries at 400 GeV along +z, 5,000 muons
+(PID 13) and 5,000 electrons, in lead, cutoff 0.100 GeV, the 400 GeV table
+set, m_V = 10 MeV.
+
+* Parity: a 400 GeV muon shower and a 150 GeV electron shower against the
+  oracle on the same Philox streams and maps (trees identical, p0 to 2e-5 of
+  each particle's energy), so the energies above the 100 GeV set, the muon
+  processes and the maps the 400 GeV set lacks (the GPU-trained pack, which
+  the oracle is given as its maps) are covered particle by particle.
+* Full size: the 10,000-primary record through the size-independent
+  properties of tests/record_checks.py, and its dark yield with dY/dm_V
+  against central differences (step 1e-5 m_V, at masses between the trained
+  ones: at a trained mass the interpolated yield has a kink).
+"""
+import math
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+import record_checks
+from oracle.shower import OracleShower
+from oracle.tables import SMTables
+from petite_amd import Particle, Shower
+from petite_amd.batch import PRIMARY_DTYPE
+from petite_amd.constants import mass_dict
+from petite_amd.dark_shower import DarkShower
+
+CUT, MV = 0.100, 0.01
+N = 10_000
+
+
+def _prim(E, pid):
+    m = mass_dict[pid]
+    return Particle([E, 0.0, 0.0, math.sqrt(E * E - m * m)], [0, 0, 0], {"PID": pid, "mass": m})
+
+
+def test_400GeV_showers_match_oracle():
+    sh = Shower("400GeV", "lead", CUT, seed=7)
+    assert sh._tables.map_source["Brem"] == "trained" and sh._tables.map_source["MuonBrem"] == "trained"
+    cases = [(400.0, 13), (150.0, 11)]
+    batch = sh.generate_showers([_prim(E, p) for E, p in cases], seed=7)
+    ctx = sh.device_context()
+    maxf = {(P, ie): v for P, arr in ctx.maxf.items() for ie, v in enumerate(arr)}
+    osh = OracleShower(SMTables("lead", "400GeV", grids=sh._tables.grids), CUT, seed=7, maxf=maxf)
+    a = batch.to_numpy()
+    for s, (E, pid) in enumerate(cases):
+        m = mass_dict[pid]
+        ref = osh.generate_shower(pid, [E, 0, 0, math.sqrt(E * E - m * m)], sid=s, shower=s)
+        sel = np.nonzero(a["shower"] == s)[0]
+        gsid = {int(a["sid"][i]): i for i in sel}
+        assert len(sel) == len(ref), (s, len(sel), len(ref))
+        worst = 0.0
+        for q in ref:
+            i = gsid[int(q.sid)]
+            assert a["pid"][i] == q.pid and (a["meta"][i] & 0xFFFF) == q.gen
+            worst = max(worst, float(np.max(np.abs(a["p0"][i] - q.p0))) / q.p0[0])
+        assert worst <= 2e-5, (s, worst)
+    assert batch.stats["n_draws_proc"][7] > 0 and batch.stats["n_draws_proc"][6] > 0  # MuonBrem, MuonE
+
+
+@pytest.fixture(scope="module")
+def full():
+    ds = DarkShower("400GeV", "lead", CUT, MV, seed=5)
+    prim = np.zeros(N, dtype=PRIMARY_DTYPE)
+    pid = np.where(np.arange(N) < N // 2, 13, 11).astype(np.int32)
+    m = np.array([mass_dict[int(p)] for p in pid])
+    prim["E"] = 400.0
+    prim["pz"] = np.sqrt(400.0 ** 2 - m * m)
+    prim["mass"], prim["weight"], prim["pid"] = m, 1.0, pid
+    prim["shower"] = np.arange(N, dtype=np.int32)
+    prim["sid"] = np.arange(N, dtype=np.uint64)
+    b = ds.generate_showers(prim, seed=5)
+    return ds, b, prim
+
+
+def test_full_size_record_properties(full):
+    ds, b, prim = full
+    assert b.n > 5e7
+    record_checks.finite_and_on_shell(b)
+    record_checks.links_generations_and_species(b, N, CUT)
+    record_checks.vertex_energy_conservation(b, 1_000_000)
+    record_checks.budget_and_counts(b, prim)
+
+
+def test_full_size_yield_gradient(full):
+    ds, b, _ = full
+    mV = torch.tensor([0.005, 0.013, 0.05, 0.2], dtype=torch.float64, requires_grad=True)
+    Y = ds.dark_yield(b, mV, 1.0)
+    (g,) = torch.autograd.grad(Y.sum(), mV)
+    assert bool(torch.all(Y > 0))
+    model = ds._yield_model
+    h = 1e-5
+    mv = mV.detach().numpy()
+    Yp, _ = model.evaluate(b, mv * (1 + h), grad=False)
+    Ym, _ = model.evaluate(b, mv * (1 - h), grad=False)
+    np.testing.assert_allclose(g.numpy(), (Yp - Ym) / (2 * h * mv), rtol=1e-4)

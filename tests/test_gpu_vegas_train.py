@@ -106,15 +106,21 @@ def test_shower_after_retraining():
 
 
 def test_missing_400gev_map_trained_at_its_energies():
-    # the 400 GeV set lacks the Brem map: the 100 GeV map (to 100 GeV) stands in
-    # until train_maps puts it on the 400 GeV cross-section energies and trains it
+    # the 400 GeV set lacks the Brem map: the shipped GPU-trained pack
+    # (petite_amd.tablegen.train_missing_maps) puts it on the 400 GeV
+    # cross-section energies, and its E[J f] reproduces the cross-section
+    # table above the 100 GeV maps' range; train_maps retrains it in place
     sh = Shower("400GeV", "lead", 0.100, seed=4)
-    E_old = sh.device_context().tables_grid("Brem")[0]
-    assert E_old[-1] <= 100.0 + 1e-9
-    sh.train_maps(["Brem"], energies="sigma", n_iter=5, n_per_energy=1 << 14)
+    assert sh._tables.map_source["Brem"] == "trained"
     ctx = sh.device_context()
     E = ctx.tables_grid("Brem")[0]
     xs = sh._tables.xsec["Brem"]
+    assert len(E) == len(xs) and E[-1] > 399.0
+    hi = E > 100.0
+    np.testing.assert_allclose(np.median(ctx.sigma_est["Brem"][hi] / xs[hi, 1]), 1.0, atol=0.05)
+    sh.train_maps(["Brem"], energies="sigma", n_iter=5, n_per_energy=1 << 14)
+    ctx = sh.device_context()
+    E = ctx.tables_grid("Brem")[0]
     assert len(E) == len(xs) and E[-1] > 399.0
     assert int(ctx.desc.grid[all_process_code["Brem"]].n_E) == len(E)
     # sigma identity above the stand-in's range: E[J f] of the trained map
@@ -128,12 +134,13 @@ def test_missing_400gev_map_trained_at_its_energies():
 
 
 def test_missing_400gev_dark_brem_map_trained_at_its_energies():
-    # the 400 GeV set lacks the DarkBrem map too: the 100 GeV map of the same
-    # mass stands in; put on the 400 GeV dark cross-section energies and
-    # trained, E[J f] above 100 GeV reproduces the shipped DarkBrem table
+    # the 400 GeV set lacks the DarkBrem map too: the shipped GPU-trained
+    # pack has it on the 400 GeV dark cross-section energies; retrained in
+    # place, E[J f] above 100 GeV reproduces the shipped DarkBrem table
     from petite_amd.dark_shower import DarkShower
     ds = DarkShower("400GeV", "lead", 0.100, 0.03, seed=5)
-    assert ds.device_context().tables_grid("DarkBrem")[0][-1] <= 100.0 + 1e-9
+    assert ds._dark.map_source["DarkBrem"] == "trained"
+    assert ds.device_context().tables_grid("DarkBrem")[0][-1] > 399.0
     ds.train_maps(["DarkBrem"], energies="sigma", n_iter=5, n_per_energy=1 << 14)
     ctx = ds.device_context()
     E = ctx.tables_grid("DarkBrem")[0]

@@ -2,8 +2,11 @@
 oracle, central finite differences and torch autograd.  Record: lead, e-
 primaries with E log-uniform in [1, 20] GeV, cutoff 0.010 GeV; masses
 geomspace(1, 100) MeV.  Tolerances: 1e-9 relative against the oracle (both
-float64 on the same float32 record values); 1e-4 against finite differences
-with step 1e-3 m_V away from the trained masses."""
+float64 on the same float32 record values); 1e-4 against central finite
+differences with step 1e-5 m_V away from the trained masses (between two
+trained masses a particle's weight goes as a power of m_V, up to m_V^-30
+where a resonance moves past it, so a step of 1e-3 m_V would leave a
+truncation error near 1e-4)."""
 import math
 
 import numpy as np
@@ -35,8 +38,8 @@ def run():
     ds.dark_yield(batch, mV[:1], 1.0)  # builds the yield model
     model = ds._yield_model
     T = SMTables("lead")
-    samplers = [OracleDarkSampler(OracleDark(T, m, with_integrals=False), model.dark[m].W, model.dark[m].D, None)
-                for m in model.masses]
+    samplers = [OracleDarkSampler(OracleDark(T, m, with_integrals=False, masses="all"), model.dark[m].W,
+                                  model.dark[m].D, None) for m in model.masses]
     a = batch.to_numpy()
     return ds, batch, model, samplers, a
 
@@ -56,7 +59,7 @@ def test_gradient_matches_finite_differences(run):
     inner = [k for k, m in enumerate(MV) if all(abs(m / t - 1) > 2e-3 for t in model.masses) and m > model.masses[0]]
     mv = MV[inner]
     Y, dY = model.evaluate(batch, mv)
-    h = 1e-3
+    h = 1e-5
     Yp, _ = model.evaluate(batch, mv * (1 + h))
     Ym, _ = model.evaluate(batch, mv * (1 - h))
     fd = (Yp - Ym) / (2 * h * mv)
