@@ -174,6 +174,20 @@ int petite_set_maxf(PetiteCtx* ctx, int32_t process, const double* maxf_host, in
 int petite_vegas_draw(PetiteCtx* ctx, const int32_t* proc, const double* E, const uint64_t* sid, int64_t n,
                       uint64_t seed, float* x_out, int32_t* trials_out, float* w_out, PetiteStream stream);
 
+/* petite_vegas_draw without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and error codes as petite_vegas_draw; the request
+ * regions and counters live in one static device array per GPU (2.4 MB, held
+ * until the process ends) instead of context scratch, and the draw kernels run
+ * one after another on `stream`.  At most PETITE_NOALLOC_MAX_DRAWS requests
+ * per call (larger n: PETITE_ERR_ARG).  Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other threads, other
+ * contexts) race and must not be issued. */
+#define PETITE_NOALLOC_MAX_DRAWS 16384
+int petite_vegas_draw_noalloc(PetiteCtx* ctx, const int32_t* proc, const double* E, const uint64_t* sid, int64_t n,
+                              uint64_t seed, float* x_out, int32_t* trials_out, float* w_out, PetiteStream stream);
+
 /* J*f of single map candidates: for request i, the map of proc[i] at
  * LU_Key(E[i]) applied to the four Philox words words[4i..4i+3] (the same
  * words a draw consumes), and the integrand at the true energy E[i].  The

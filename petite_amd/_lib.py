@@ -21,8 +21,9 @@ PETITE_ERR_INTERNAL = 5
 EXPORTED = ["petite_ctx_create", "petite_ctx_destroy", "petite_last_error", "petite_maxf_scan", "petite_set_maxf",
             "petite_vegas_draw", "petite_eval_points", "petite_run_showers", "petite_set_timing", "petite_dark_setup",
             "petite_dark_reweight", "petite_yield_setup", "petite_dark_yield", "petite_detector_cut",
-            "petite_map_accumulate", "petite_set_grid", "petite_set_tuning", "petite_get_tuning", "petite_sample_scattering",
+            "petite_map_accumulate", "petite_set_grid", "petite_set_tuning", "petite_get_tuning", "petite_vegas_draw_noalloc", "petite_sample_scattering",
             "petite_propagate", "petite_draw_pe"]
+NOALLOC_MAX_DRAWS = 16384
 MAX_MASSES = 16
 DARK_FAMILIES = ["brem_elec", "brem_positron", "muon_brem", "annihilation"]
 DARK_DRATE_BINS = 10
@@ -161,6 +162,8 @@ def load():
     lib.petite_set_maxf.restype = C.c_int
     lib.petite_vegas_draw.argtypes = [vp, vp, vp, vp, i64, u64, vp, vp, vp, vp]
     lib.petite_vegas_draw.restype = C.c_int
+    lib.petite_vegas_draw_noalloc.argtypes = [vp, vp, vp, vp, i64, u64, vp, vp, vp, vp]
+    lib.petite_vegas_draw_noalloc.restype = C.c_int
     lib.petite_eval_points.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp]
     lib.petite_eval_points.restype = C.c_int
     lib.petite_run_showers.argtypes = [vp, vp, i64, u64, C.POINTER(Store), vp, C.POINTER(Stats), vp]

@@ -1312,6 +1312,29 @@ __global__ __launch_bounds__(256) void k_detector_cut(const float4* __restrict__
       if (s_w[d] != 0.0) unsafeAtomicAdd(wsum + d, s_w[d]);
 }
 
+// --------------------------------------------------- allocation-free draws --
+// petite_vegas_draw_noalloc keeps its request regions and counters in one
+// file-scope device array instead of context scratch: [PETITE_NPROC][kNoAllocCap]
+// request slots (int32) and energies (float64), then the counter block
+// (C_N uint64) and the per-process request counts (uint32).  Every GPU the
+// library runs on holds one copy until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress returns on that device.
+constexpr int64_t kNoAllocCap = PETITE_NOALLOC_MAX_DRAWS;
+constexpr size_t kNoAllocSlotOff = 0;
+constexpr size_t kNoAllocEOff = kNoAllocSlotOff + sizeof(int32_t) * PETITE_NPROC * kNoAllocCap;
+constexpr size_t kNoAllocCtrOff = kNoAllocEOff + sizeof(double) * PETITE_NPROC * kNoAllocCap;
+constexpr size_t kNoAllocCountOff = kNoAllocCtrOff + sizeof(unsigned long long) * C_N;
+constexpr size_t kNoAllocBytes = (kNoAllocCountOff + sizeof(unsigned int) * PETITE_NPROC + 255) / 256 * 256;
+__device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Clears the counter block and the request counts of the allocation-free
+// draws (a kernel rather than a memset, so the call stays capturable).
+__global__ void k_noalloc_clear(unsigned long long* ctr, unsigned int* counts) {
+  const int t = threadIdx.x;
+  if (t < C_N) ctr[t] = 0ull;
+  if (t < PETITE_NPROC) counts[t] = 0u;
+}
+
 }  // namespace
 
 // ===================================================================== C ABI
@@ -1348,6 +1371,7 @@ struct PetiteCtx {
   std::vector<hipEvent_t> tev;
   int n_cu = 256;
   PetiteTuning tune{};  // launch and queue sizes in effect (defaults filled in)
+  unsigned char* noalloc = nullptr;  // this device's copy of `scratch` (hipGetSymbolAddress)
   double tbar[PETITE_NPROC];  // running mean of candidates per draw, per process
   // dark sector
   bool has_dark = false;
@@ -1551,6 +1575,36 @@ static int launch_draws_device(PetiteCtx* c, const DrawLaunch& io, uint64_t seed
   return PETITE_OK;
 }
 
+// Device address of `scratch` on each device, looked up once per device with
+// that device current (the array's host-side name is not a device address).
+static std::mutex g_noalloc_mu;
+static unsigned char* g_noalloc_ptr[64] = {};
+
+static int noalloc_scratch(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= 64) return fail(PETITE_ERR_ARG, "noalloc scratch: device index above 63");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(PETITE_ERR_HIP, "noalloc scratch: hipGetSymbolAddress returned a null address");
+    g_noalloc_ptr[dev] = (unsigned char*)p;
+  }
+  *out = g_noalloc_ptr[dev];
+  return PETITE_OK;
+}
+
+// Draw kernels of the active processes one after another on the caller's
+// stream (no per-process streams: creating them would allocate).
+static int launch_draws_serial(PetiteCtx* c, const DrawLaunch& io, uint64_t seed, hipStream_t s) {
+  for (int p = 0; p < PETITE_NPROC; ++p) {
+    const int b = io.block_start[p + 1] - io.block_start[p];
+    if (b <= 0 || io.count[p] <= 0 || c->desc.grid[p].n_E <= 0) continue;
+    hipLaunchKernelGGL(kDrawKernels[p], dim3((unsigned)b), dim3(256), 0, s, c->dev, io, seed);
+    HIP_TRY(hipGetLastError());
+  }
+  return PETITE_OK;
+}
+
 extern "C" {
 
 const char* petite_last_error(void) { return g_err.c_str(); }
@@ -1647,6 +1701,10 @@ int petite_ctx_create(const PetiteTableDesc* desc, const double* blob64, int64_t
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
     c->n_cu = prop.multiProcessorCount;
   c->tune = default_tuning();
+  {
+    const int rc = noalloc_scratch(dev, &c->noalloc);
+    if (rc) return cleanup(rc);
+  }
   *out = c;
   return PETITE_OK;
 }
@@ -1810,6 +1868,49 @@ int petite_vegas_draw(PetiteCtx* c, const int32_t* proc, const double* E, const 
   HIP_TRY(hipMemcpyAsync(st, c->counters + C_STATS, sizeof(st), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (st[ST_NOSAMPLE]) return fail(PETITE_ERR_NO_SAMPLE, "petite_vegas_draw: No Sample Found within max_trials");
+  return PETITE_OK;
+}
+
+int petite_vegas_draw_noalloc(PetiteCtx* c, const int32_t* proc, const double* E, const uint64_t* sid, int64_t n,
+                              uint64_t seed, float* x_out, int32_t* trials_out, float* w_out, PetiteStream stream) {
+  if (!c || n < 0 || (n > 0 && (!proc || !E || !sid || !x_out || !trials_out || !w_out)))
+    return fail(PETITE_ERR_ARG, "petite_vegas_draw_noalloc: bad arguments");
+  if (n > kNoAllocCap)
+    return fail(PETITE_ERR_ARG, "petite_vegas_draw_noalloc: more than PETITE_NOALLOC_MAX_DRAWS (16384) requests");
+  if (n == 0) return PETITE_OK;
+  if (!c->noalloc) return fail(PETITE_ERR_HIP, "petite_vegas_draw_noalloc: no scratch address for this device");
+  hipStream_t s = (hipStream_t)stream;
+  int32_t* req_slot = (int32_t*)(c->noalloc + kNoAllocSlotOff);
+  double* req_E = (double*)(c->noalloc + kNoAllocEOff);
+  unsigned long long* ctr = (unsigned long long*)(c->noalloc + kNoAllocCtrOff);
+  unsigned int* counts = (unsigned int*)(c->noalloc + kNoAllocCountOff);
+  // the previous call's counters are still there: clear what must start at zero
+  hipLaunchKernelGGL(k_noalloc_clear, dim3(1), dim3(128), 0, s, ctr, counts);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_bucket, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, c->dev, proc, E, n, req_slot, req_E,
+                     kNoAllocCap, counts, trials_out, w_out);
+  HIP_TRY(hipGetLastError());
+  unsigned int rq[PETITE_NPROC];
+  HIP_TRY(hipMemcpyAsync(rq, counts, sizeof(rq), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  DrawLaunch io{};
+  io.req_slot = req_slot;
+  io.req_E = req_E;
+  io.sid_src = sid;
+  io.req_cap = kNoAllocCap;
+  io.explicit_out = 1;
+  io.x_out = x_out;
+  io.trials_out = trials_out;
+  io.w_out = w_out;
+  io.counters = ctr;
+  int64_t blocks = 0;
+  plan_draw(c, rq, io, blocks);
+  int rc = launch_draws_serial(c, io, seed, s);
+  if (rc) return rc;
+  unsigned long long st[ST_N];
+  HIP_TRY(hipMemcpyAsync(st, ctr + C_STATS, sizeof(st), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (st[ST_NOSAMPLE]) return fail(PETITE_ERR_NO_SAMPLE, "petite_vegas_draw_noalloc: No Sample Found within max_trials");
   return PETITE_OK;
 }
 

@@ -167,8 +167,10 @@ class DeviceContext:
         _lib.check(self.lib.petite_set_maxf(self.ctx, code, v.ctypes.data, v.size), "petite_set_maxf")
         self.maxf[process] = v.copy()
 
-    def draw(self, proc, E, sid, seed):
-        """Batched draws; proc/E/sid are 1-D host arrays or cuda tensors."""
+    def draw(self, proc, E, sid, seed, noalloc=False):
+        """Batched draws; proc/E/sid are 1-D host arrays or cuda tensors.
+        noalloc=True uses petite_vegas_draw_noalloc (no device allocation by
+        the library; at most _lib.NOALLOC_MAX_DRAWS requests)."""
         dev = torch.device("cuda")
         proc_t = torch.as_tensor(np.asarray(proc, dtype=np.int32) if not torch.is_tensor(proc) else proc,
                                  device=dev, dtype=torch.int32).contiguous()
@@ -184,10 +186,11 @@ class DeviceContext:
         x = torch.empty((n, 4), device=dev, dtype=torch.float32)
         trials = torch.empty(n, device=dev, dtype=torch.int32)
         w = torch.empty(n, device=dev, dtype=torch.float32)
-        rc = self.lib.petite_vegas_draw(self.ctx, proc_t.data_ptr(), E_t.data_ptr(), sid_t.data_ptr(), n, int(seed), x.data_ptr(),
+        fn = self.lib.petite_vegas_draw_noalloc if noalloc else self.lib.petite_vegas_draw
+        rc = fn(self.ctx, proc_t.data_ptr(), E_t.data_ptr(), sid_t.data_ptr(), n, int(seed), x.data_ptr(),
                 trials.data_ptr(), w.data_ptr(), _stream_handle())
         if rc not in (_lib.PETITE_OK, _lib.PETITE_ERR_NO_SAMPLE):
-            _lib.check(rc, "petite_vegas_draw")
+            _lib.check(rc, "petite_vegas_draw_noalloc" if noalloc else "petite_vegas_draw")
         return x, trials, w, rc
 
     def sample_scattering(self, vertices, proc, seed):
