@@ -254,6 +254,50 @@ int petite_draw_pe(const uint64_t* sid, int64_t n, uint64_t seed, double Zeff, d
 #define PETITE_DARK_FAMILIES 4
 #define PETITE_DARK_DRATE_BINS 10
 
+/* Track-length quadrature of the dark tables.  Replaces the scipy quad
+ * integrals of DarkShower's weight arrays (construct_brem_weight_array,
+ * construct_annihilation_weight_array, construct_muon_brem_weight_array,
+ * src/PETITE/dark_shower.py:337-399) and rate rows (_d_rate_d_E_*, 454-533),
+ * with the survival factor of the SM interaction integrals
+ * (src/PETITE/shower.py:298-354).  Each job integrates
+ *   n sigma(E) / (dE/dx) * exp(-sum_c [II_c(Ei) - II_c(E)] / (dE/dx))
+ * over E (cm units as the reference: n sigma in 1/cm, dE/dx in GeV/cm), with
+ * II_c the running integral of survival channel c's n sigma on its grid,
+ * either from `lo` to each end energy Ei (mode 0, the weight W(Ei)) or over
+ * ten equal bins of [max(Ei - 10 mfp(Ei) dE/dx, lo), Ei] (mode 1, the drate
+ * row: bin centre and integral per bin).  Tables are offsets into one float64
+ * blob: x values at off, y values at off + n.  Gauss-Legendre panels (8
+ * points) break at every node of `nodes` and at geometric distances below Ei
+ * that follow the survival factor; results agree with adaptive quadrature to
+ * the reference tables' precision. */
+typedef struct {
+  int64_t off; /* offset (doubles) of x in the blob; y (if any) at off + n */
+  int32_t n;   /* points */
+  int32_t pad;
+} PetiteQTable;
+
+typedef struct {
+  PetiteQTable ns;       /* log10 E, log10 n*sigma [1/cm] of the dark process (log-log, 1e-20 outside) */
+  PetiteQTable loss[3];  /* n*sigma (x, y) of each survival channel (linear, 0 outside) */
+  PetiteQTable grid[3];  /* grid (x only) each channel's running integral is kept on */
+  PetiteQTable mfp[3];   /* n*sigma (x, y) tables summed for the mean free path (get_mfp) */
+  PetiteQTable nodes;    /* panel nodes (x only), ascending and distinct */
+  PetiteQTable energies; /* end energies Ei (x only) */
+  int32_t n_loss, n_mfp; /* channels used, each <= 3 */
+  int32_t mode;          /* 0: weights; 1: drate rows */
+  int32_t zero_small;    /* 1: n*sigma below 1e-18 counts as 0 (all but bound-electron DarkAnn) */
+  double lo;             /* mode 0: lower limit; mode 1: floor of the bins */
+  double dEdx;           /* GeV/m */
+  int64_t out_off;       /* mode 0: n values W(Ei); mode 1: n x 10 x (bin centre, rate) */
+} PetiteDarkQuadJob;
+
+/* Runs the jobs on the current device (synchronous; blob and out are host
+ * arrays).  PETITE_ERR_ARG for tables outside the blob or results outside
+ * out, PETITE_ERR_INTERNAL if an integral needs more than 512 geometric panel
+ * edges. */
+int petite_dark_quadrature(const PetiteDarkQuadJob* jobs, int32_t n_jobs, const double* blob, int64_t n_blob,
+                           double* out, int64_t n_out);
+
 /* Dark tables of one (material, m_V) pair, built by DarkShower.__init__
  * (src/PETITE/dark_shower.py:85-593).  Offsets index the float64 blob given to
  * petite_dark_setup. */
@@ -331,7 +375,8 @@ int petite_dark_reweight(PetiteCtx* ctx, const PetiteStore* sm, int64_t n_sm, ui
  * W_P = GetBSMWeights (src/PETITE/dark_shower.py:595-647) with eps^2 = 1.  For
  * m_V between trained masses m_j < m_{j+1}, with t = ln(m_V/m_j)/ln(m_{j+1}/m_j),
  * each particle's weight is W_j^(1-t) W_{j+1}^t when both are positive and
- * (1-t) W_j + t W_{j+1} otherwise.  Masses outside [lightest, heaviest
+ * (1-t) W_j + t W_{j+1} otherwise (DarkAnn and DarkComp: tables at m_V
+ * itself with petite_yield_bound).  Masses outside [lightest, heaviest
  * trained mass] are rejected (PETITE_ERR_ARG): there is nothing to
  * interpolate. */
 #define PETITE_MAX_MASSES 16
@@ -341,6 +386,32 @@ int petite_dark_reweight(PetiteCtx* ctx, const PetiteStore* sm, int64_t n_sm, ui
  * be 1.  The context must hold the SM n*sigma tables of the same material. */
 int petite_yield_setup(PetiteCtx* ctx, int32_t n_mass, const double* masses, const PetiteDarkDesc* descs,
                        const double* blob, int64_t n);
+
+/* Bound-electron processes of the yield.  With these inputs set (after
+ * petite_yield_setup, which clears them), the DarkAnn and DarkComp weights at
+ * each requested m_V come from tables built at that m_V on the device, as
+ * DarkShower builds them for its own mass (src/PETITE/dark_shower.py:254-292
+ * with the closed forms of src/PETITE/atomic_annihilation.py:70-104 and
+ * src/PETITE/atomic_compton.py:94-109, the DarkAnn track-length weights as
+ * petite_dark_quadrature computes them), instead of interpolation between
+ * trained masses: their weights are resonances and thresholds that move
+ * with m_V.  Their dY/dm_V is the central difference of tables built at
+ * m_V (1 +- rel_step).  Offsets index the float64 blob (host, n doubles). */
+typedef struct {
+  double Zeff;                         /* effective charge of the bound-electron forms */
+  double ne_conv;                      /* n_e * GeVsqcm2: n*sigma [1/cm] per sigma [1/GeV^2] */
+  double m_e, alpha;                   /* m_electron [GeV] and alpha_em (Compton form, thresholds) */
+  double ann_floor, comp_floor;        /* minimum calculable e+ / photon energies: lower grid ends */
+  double rel_step;                     /* relative step of the central difference (0 < h < 0.01) */
+  double dEdx;                         /* GeV/m */
+  double top_ann[PETITE_MAX_MASSES];   /* per trained mass: top energy of its grids, used for m_V at or */
+  double top_comp[PETITE_MAX_MASSES];  /* above it (DarkBrem and free DarkComp tables' last energies) */
+  PetiteQTable loss[3];                /* positron survival channels Brem, Ann, Bhabha: n*sigma (x, y) */
+  PetiteQTable mfp[3];                 /* mean-free-path tables (x, y): Brem, Bhabha, Ann */
+  PetiteQTable nodes;                  /* panel nodes of the survival grids (x only; ascending, distinct) */
+} PetiteBoundDesc;
+
+int petite_yield_bound(PetiteCtx* ctx, const PetiteBoundDesc* desc, const double* blob, int64_t n);
 
 /* Y and dY/dm_V at n_points masses (host arrays), accumulated in float64 over
  * the first n_sm records of the store.  dyield_dmV_host may be NULL: the

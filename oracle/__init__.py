@@ -35,6 +35,9 @@ Parity pins (what checks each function; see tests/):
   lumi_integral_list (test_oracle_dark_ann_lumi.py).
 * transport: exponential free-path law (KS), per-shower energy budget.
 * dark weights/drate: the stored ``dark_weights.pkl``/``dark_drate.pkl``
-  caches and quadrature (test_dark_tables.py, test_oracle_dark_yield.py).
+  caches and scipy quad (test_dark_tables.py, test_oracle_dark_yield.py); the
+  panel rule of dark_panels.py against the same quad integrals and caches,
+  and its running integrals against closed-form trapezoids
+  (test_dark_tables.py).
 Functions without such a check say "parity unpinned" in their docstring.
 """

@@ -465,12 +465,66 @@ def sampler_weights(s, pid, E0, process):
     return out
 
 
-def yield_and_grad(samplers, masses, processes, pid, E0, w, mV):
+BOUND_PROCESSES = ("DarkAnn", "DarkComp")
+
+
+def bound_tables(T, mV, table_set="100GeV", Zeff=29.508, masses="all"):
+    """The bound-electron tables DarkShower builds for its own mass
+    (dark_shower.py:254-292, mode "approx": cross sections at mV itself, grid
+    tops from the trained mass at or below it) and the DarkAnn track-length
+    weights by the panel rule (oracle.dark_panels)."""
+    from .dark_panels import Panels
+    od = OracleDark(T, mV, mode="approx", Zeff=Zeff, table_set=table_set, with_integrals=False, masses=masses)
+    return od, Panels(od).weight_array("DarkAnn", -11)
+
+
+def bound_weights(od, Wa, pid, E0, process):
+    """GetBSMWeights of DarkAnn / DarkComp (595-647) with eps^2 = 1 from bound_tables."""
+    pid = np.asarray(pid)
+    E0 = np.asarray(E0, dtype=np.float64)
+    T = od.T
+    out = np.zeros(E0.shape)
+    if process == "DarkComp":
+        m = (pid == 22) & (E0 >= od.min_dark[22]["DarkComp"]) & (E0 >= T.min_calc[22])
+        if m.any():
+            e = E0[m]
+            out[m] = od.NS["DarkComp"](e) / (T.nsigma("PairProd", e) + T.nsigma("Comp", e))
+        return out
+    m = (pid == -11) & (E0 >= od.min_dark[-11]["DarkAnn"])
+    if m.any():
+        out[m] = interp0(Wa[:, 0], Wa[:, 1], E0[m])
+    return out
+
+
+def bound_yield(T, processes, pid, E0, w, mV, table_set="100GeV", rel_step=1e-5, Zeff=29.508, masses="all"):
+    """Y and dY/dm_V of the bound-electron processes with tables at each m_V
+    itself; dY/dm_V is the central difference of tables at m_V (1 +- rel_step)."""
+    procs = [P for P in processes if P in BOUND_PROCESSES]
+    w = np.asarray(w, dtype=np.float64)
+    Y, dY = [], []
+    for m in np.atleast_1d(mV):
+        vals = []
+        for mm in (m, m * (1 + rel_step), m * (1 - rel_step)):
+            od, Wa = bound_tables(T, mm, table_set, Zeff, masses)
+            vals.append(float(np.sum(sum(bound_weights(od, Wa, pid, E0, P) for P in procs) * w)) if procs else 0.0)
+        Y.append(vals[0])
+        dY.append((vals[1] - vals[2]) / (2 * rel_step * m))
+    return np.array(Y), np.array(dY)
+
+
+def yield_and_grad(samplers, masses, processes, pid, E0, w, mV, bound_T=None, table_set="100GeV"):
     """Y(m_V) = sum_i w_i sum_P W_P(E0_i; m_V) and dY/dm_V with each particle's
     weight interpolated between the trained masses: geometric in
     t = ln(m_V/m_j)/ln(m_{j+1}/m_j) where both weights are positive, linear
     otherwise; t clamped to [0, 1] (zero derivative) outside the masses.
-    samplers[j] evaluates the weights at masses[j] with eps^2 = 1."""
+    samplers[j] evaluates the weights at masses[j] with eps^2 = 1.  With
+    bound_T (the SM tables), DarkAnn and DarkComp use tables at each m_V
+    instead (bound_yield)."""
+    if bound_T is not None:
+        rest = [P for P in processes if P not in BOUND_PROCESSES]
+        Yb, dYb = bound_yield(bound_T, processes, pid, E0, w, mV, table_set)
+        Yr, dYr = yield_and_grad(samplers, masses, rest, pid, E0, w, mV) if rest else (0.0, 0.0)
+        return Yb + Yr, dYb + dYr
     masses = np.asarray(masses, dtype=np.float64)
     wts = np.array([[sampler_weights(samplers[j], pid, E0, P) for P in processes] for j in range(len(masses))])
     w = np.asarray(w, dtype=np.float64)

@@ -22,7 +22,7 @@ EXPORTED = ["petite_ctx_create", "petite_ctx_destroy", "petite_last_error", "pet
             "petite_vegas_draw", "petite_eval_points", "petite_run_showers", "petite_set_timing", "petite_dark_setup",
             "petite_dark_reweight", "petite_yield_setup", "petite_dark_yield", "petite_detector_cut",
             "petite_map_accumulate", "petite_set_grid", "petite_set_tuning", "petite_get_tuning", "petite_vegas_draw_noalloc", "petite_sample_scattering",
-            "petite_propagate", "petite_draw_pe"]
+            "petite_propagate", "petite_draw_pe", "petite_dark_quadrature", "petite_yield_bound"]
 NOALLOC_MAX_DRAWS = 16384
 MAX_MASSES = 16
 DARK_FAMILIES = ["brem_elec", "brem_positron", "muon_brem", "annihilation"]
@@ -103,6 +103,25 @@ class Tuning(C.Structure):
         return t
 
 
+class QTable(C.Structure):
+    _fields_ = [("off", C.c_int64), ("n", C.c_int32), ("pad", C.c_int32)]
+
+
+class DarkQuadJob(C.Structure):
+    """PetiteDarkQuadJob: one weight or drate job of petite_dark_quadrature."""
+    _fields_ = [("ns", QTable), ("loss", QTable * 3), ("grid", QTable * 3), ("mfp", QTable * 3), ("nodes", QTable),
+                ("energies", QTable), ("n_loss", C.c_int32), ("n_mfp", C.c_int32), ("mode", C.c_int32),
+                ("zero_small", C.c_int32), ("lo", C.c_double), ("dEdx", C.c_double), ("out_off", C.c_int64)]
+
+
+class BoundDesc(C.Structure):
+    """PetiteBoundDesc: bound-electron inputs of the m_V-continuous yield."""
+    _fields_ = [("Zeff", C.c_double), ("ne_conv", C.c_double), ("m_e", C.c_double), ("alpha", C.c_double),
+                ("ann_floor", C.c_double), ("comp_floor", C.c_double), ("rel_step", C.c_double), ("dEdx", C.c_double),
+                ("top_ann", C.c_double * 16), ("top_comp", C.c_double * 16), ("loss", QTable * 3),
+                ("mfp", QTable * 3), ("nodes", QTable)]
+
+
 class DarkDesc(C.Structure):
     _fields_ = [("w_n", C.c_int32 * 4), ("w_off", C.c_int64 * 4), ("d_n", C.c_int32 * 4), ("d_off", C.c_int64 * 4),
                 ("comp_n", C.c_int32), ("n_active", C.c_int32), ("comp_off", C.c_int64), ("active", C.c_int32 * 8),
@@ -168,6 +187,10 @@ def load():
     lib.petite_eval_points.restype = C.c_int
     lib.petite_run_showers.argtypes = [vp, vp, i64, u64, C.POINTER(Store), vp, C.POINTER(Stats), vp]
     lib.petite_run_showers.restype = C.c_int
+    lib.petite_yield_bound.argtypes = [vp, C.POINTER(BoundDesc), vp, i64]
+    lib.petite_yield_bound.restype = C.c_int
+    lib.petite_dark_quadrature.argtypes = [C.POINTER(DarkQuadJob), i32, vp, i64, vp, i64]
+    lib.petite_dark_quadrature.restype = C.c_int
     lib.petite_dark_setup.argtypes = [vp, C.POINTER(DarkDesc), vp, i64]
     lib.petite_dark_setup.restype = C.c_int
     lib.petite_dark_reweight.argtypes = [vp, C.POINTER(Store), i64, u64, C.POINTER(DarkOut), C.POINTER(DarkStats), vp]

@@ -610,13 +610,47 @@ __device__ inline void yield_weight(const DevTables& T, const PetiteDarkDesc& D,
   lam = wv > 0.0 ? log(wv) : 0.0;
 }
 
+// Bound-electron tables at the pass's masses (quad.h): slot k * nv + v holds
+// m_k (v = 0), m_k (1 + h) (v = 1) and m_k (1 - h) (v = 2, with the
+// derivative).  slot == nullptr: DarkAnn and DarkComp interpolate between the
+// trained masses like the other processes.
+struct BoundView {
+  const quad::BoundSlot* slot;
+  const double* tab;
+  int32_t stride;  // doubles per slot
+  int32_t nv;
+  double inv_2h[kYieldBatch];  // 1 / (2 h m_k)
+};
+
+// GetBSMWeights of DarkAnn / DarkComp (dark_shower.py:595-647) with the tables
+// of slot s: the annihilation weight array (linear, 0 outside) and the
+// Compton log-log n sigma over n sigma_PP + n sigma_Comp.
+__device__ inline double bound_w(const DevTables& T, const BoundView& B, int s, int code, const WeightArgs& a,
+                                 double ln_den, double ln_eps2) {
+  const quad::BoundSlot& S = B.slot[s];
+  const double* t = B.tab + (int64_t)s * B.stride;
+  constexpr int n = quad::kBoundGrid;
+  if (code == DARKANN) {
+    const double E0 = a.E0;
+    if (E0 < S.min_ann || !(E0 >= t[0]) || E0 > t[n - 1]) return 0.0;
+    const double* y = t + 3 * n;
+    if (E0 == t[n - 1]) return exp(ln_eps2) * y[n - 1];
+    const int lo = find_interval(t, n, E0, a.lf, S.lnEa0, S.inv_dln_a);
+    const double slope = (y[lo + 1] - y[lo]) / (t[lo + 1] - t[lo]);
+    return exp(ln_eps2) * (slope * (E0 - t[lo]) + y[lo]);
+  }
+  if (a.E0 < S.min_comp || a.E0 < T.d->min_calc[2]) return 0.0;
+  const double L = loglog10(t + 4 * n, t + 5 * n, n, a.l10, S.l0c, S.inv_dlc);
+  return exp(ln_eps2 - ln_den + kLn10 * L);
+}
+
 // One float64 pass over the record: acc[2k] += w W(m_k), acc[2k+1] += w dW/dm(m_k)
 // (GRAD = false: the yield alone).  Between trained masses j and j+1 a
 // particle's weight is interpolated geometrically, W = exp(ln w_j + t ln(w_j+1 /
 // w_j)), where both are positive, linearly otherwise.
 template <bool GRAD>
 __global__ __launch_bounds__(256) void k_dark_yield(DevTables T, const YieldTables* __restrict__ Yp, DevStore S,
-                                                    int64_t n, YieldPoints P, double* acc) {
+                                                    int64_t n, YieldPoints P, BoundView Bv, double* acc) {
   const YieldTables& Y = *Yp;
   double y[kYieldBatch], dy[kYieldBatch];
 #pragma unroll
@@ -632,6 +666,18 @@ __global__ __launch_bounds__(256) void k_dark_yield(DevTables T, const YieldTabl
     for (int a = 0; a < d0.n_active; ++a) {
       const int code = d0.active[a];
       if (code == DARKCOMP ? pid != 22 : family_of(code, pid) < 0) continue;
+      if (Bv.slot && (code == DARKANN || code == DARKCOMP)) {
+#pragma unroll
+        for (int k = 0; k < kYieldBatch; ++k) {
+          if (k >= P.n) break;
+          const int s0 = k * Bv.nv;
+          y[k] += w * bound_w(T, Bv, s0, code, wa, ln_den, P.ln_eps2[0]);
+          if (GRAD)
+            dy[k] += w * (bound_w(T, Bv, s0 + 1, code, wa, ln_den, P.ln_eps2[0]) -
+                          bound_w(T, Bv, s0 + 2, code, wa, ln_den, P.ln_eps2[0])) * Bv.inv_2h[k];
+        }
+        continue;
+      }
       double wv[kYieldWindow], lam[kYieldWindow];
 #pragma unroll
       for (int q = 0; q < kYieldWindow; ++q) {

@@ -1199,6 +1199,7 @@ __global__ void k_sweep_setup(unsigned long long* ctr, int64_t* range, unsigned 
   }
 }
 
+#include "quad.h"
 #include "dark.h"
 
 // draw_pe_sample (dark_shower.py:706-719) for n stream ids: the bound
@@ -1389,6 +1390,18 @@ struct PetiteCtx {
   dark::YieldTables* ytab_dev = nullptr;  // device copy read by the kernel
   double* yield_blob = nullptr;
   double* yacc = nullptr;
+  // bound-electron tables at each m_V of the yield (petite_yield_bound)
+  bool bnd_on = false;
+  quad::BoundParams bnd{};
+  double bnd_h = 0.0;
+  double bnd_top_ann[PETITE_MAX_MASSES] = {};
+  double bnd_top_comp[PETITE_MAX_MASSES] = {};
+  double* bnd_blob = nullptr;          // survival and mean-free-path tables, running integrals, fixed nodes
+  double* bnd_tab = nullptr;           // per slot: grids, log-log tables, weights, panel nodes
+  quad::BoundSlot* bnd_slots = nullptr;
+  quad::Job* bnd_jobs = nullptr;
+  int* bnd_bad = nullptr;
+  const double* bnd_fixed = nullptr;   // survival-grid panel nodes (in bnd_blob)
 };
 
 static void free_entries(PetiteCtx* c) {
@@ -1726,6 +1739,7 @@ void petite_ctx_destroy(PetiteCtx* c) {
   for (void* p : {(void*)c->d64, (void*)c->d32, (void*)c->dmaxf, (void*)c->req_slot, (void*)c->req_E, (void*)c->res_x,
                   (void*)c->res_trials, (void*)c->res_w, (void*)c->req_count, (void*)c->counters, (void*)c->ddesc,
                   (void*)c->dsig_lds, (void*)c->dark_blob, (void*)c->dyield, (void*)c->yield_blob, (void*)c->yacc, (void*)c->ytab_dev,
+                  (void*)c->bnd_blob, (void*)c->bnd_tab, (void*)c->bnd_slots, (void*)c->bnd_jobs, (void*)c->bnd_bad,
                   (void*)c->par.sid, (void*)c->par.pf, (void*)c->par.pos, (void*)c->par.info})
     if (p) (void)hipFree(p);
   free_entries(c);
@@ -2191,6 +2205,112 @@ static int check_dark_desc(const PetiteDarkDesc* desc, const double* blob, int64
   return PETITE_OK;
 }
 
+int petite_dark_quadrature(const PetiteDarkQuadJob* jobs, int32_t n_jobs, const double* blob, int64_t n_blob,
+                           double* out, int64_t n_out) {
+  if (n_jobs < 0 || (n_jobs > 0 && (!jobs || !blob || !out)) || n_blob < 0 || n_out < 0)
+    return fail(PETITE_ERR_ARG, "petite_dark_quadrature: bad arguments");
+  if (n_jobs == 0) return PETITE_OK;
+  auto in_blob = [&](const PetiteQTable& t, int cols) {
+    return t.n >= 0 && t.off >= 0 && t.off + (int64_t)cols * t.n <= n_blob;
+  };
+  // device layout: blob, then the running integrals (one per job channel)
+  int64_t n_cum = 0, n_int = 0;
+  for (int j = 0; j < n_jobs; ++j) {
+    const PetiteDarkQuadJob& q = jobs[j];
+    if (q.n_loss < 0 || q.n_loss > 3 || q.n_mfp < 0 || q.n_mfp > 3 || (q.mode != 0 && q.mode != 1) || q.ns.n < 1 ||
+        !in_blob(q.ns, 2) || !in_blob(q.nodes, 1) || !in_blob(q.energies, 1) || !(q.dEdx > 0.0))
+      return fail(PETITE_ERR_ARG, "petite_dark_quadrature: bad job " + std::to_string(j));
+    for (int c = 0; c < q.n_loss; ++c)
+      if (!in_blob(q.loss[c], 2) || !in_blob(q.grid[c], 1) || q.grid[c].n < 1)
+        return fail(PETITE_ERR_ARG, "petite_dark_quadrature: bad survival channel in job " + std::to_string(j));
+    for (int c = 0; c < q.n_mfp; ++c)
+      if (!in_blob(q.mfp[c], 2)) return fail(PETITE_ERR_ARG, "petite_dark_quadrature: bad mfp table in job " + std::to_string(j));
+    const double* nd = blob + q.nodes.off;
+    for (int i = 1; i < q.nodes.n; ++i)
+      if (!(nd[i] > nd[i - 1])) return fail(PETITE_ERR_ARG, "petite_dark_quadrature: nodes must ascend strictly");
+    const int64_t per = q.mode ? 20 : 1;
+    if (q.out_off < 0 || q.out_off + per * q.energies.n > n_out)
+      return fail(PETITE_ERR_ARG, "petite_dark_quadrature: results outside out in job " + std::to_string(j));
+    for (int c = 0; c < q.n_loss; ++c) n_cum += q.grid[c].n;
+    n_int += (q.mode ? 10 : 1) * (int64_t)q.energies.n;
+  }
+  double* d_blob = nullptr;
+  double* d_out = nullptr;
+  quad::Job* d_jobs = nullptr;
+  int* d_bad = nullptr;
+  auto release = [&](int rc) {
+    (void)hipFree(d_blob);
+    (void)hipFree(d_out);
+    (void)hipFree(d_jobs);
+    (void)hipFree(d_bad);
+    return rc;
+  };
+#define QTRY(expr)                                                                                     \
+  do {                                                                                                 \
+    hipError_t _e = (expr);                                                                            \
+    if (_e != hipSuccess) return release(fail(PETITE_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e))); \
+  } while (0)
+  QTRY(hipMalloc(&d_blob, sizeof(double) * (size_t)(n_blob + n_cum + 1)));
+  QTRY(hipMalloc(&d_out, sizeof(double) * (size_t)(n_out + 1)));
+  QTRY(hipMalloc(&d_jobs, sizeof(quad::Job) * (size_t)n_jobs));
+  QTRY(hipMalloc(&d_bad, sizeof(int)));
+  std::vector<quad::Job> hj((size_t)n_jobs);
+  int64_t cum_at = n_blob, first = 0;
+  for (int j = 0; j < n_jobs; ++j) {
+    const PetiteDarkQuadJob& q = jobs[j];
+    quad::Job& J = hj[(size_t)j];
+    J.lx = d_blob + q.ns.off;
+    J.ly = d_blob + q.ns.off + q.ns.n;
+    J.n_ns = q.ns.n;
+    J.zero_small = q.zero_small ? 1 : 0;
+    J.n_loss = q.n_loss;
+    J.n_mfp = q.n_mfp;
+    for (int c = 0; c < 3; ++c) {
+      const bool on = c < q.n_loss;
+      J.loss_x[c] = on ? d_blob + q.loss[c].off : nullptr;
+      J.loss_y[c] = on ? d_blob + q.loss[c].off + q.loss[c].n : nullptr;
+      J.loss_n[c] = on ? q.loss[c].n : 0;
+      J.grid[c] = on ? d_blob + q.grid[c].off : nullptr;
+      J.grid_n[c] = on ? q.grid[c].n : 0;
+      J.cum[c] = on ? d_blob + cum_at : nullptr;
+      if (on) cum_at += q.grid[c].n;
+      const bool mon = c < q.n_mfp;
+      J.mfp_x[c] = mon ? d_blob + q.mfp[c].off : nullptr;
+      J.mfp_y[c] = mon ? d_blob + q.mfp[c].off + q.mfp[c].n : nullptr;
+      J.mfp_n[c] = mon ? q.mfp[c].n : 0;
+    }
+    J.n_nodes = q.nodes.n;
+    J.nodes = d_blob + q.nodes.off;
+    J.E = d_blob + q.energies.off;
+    J.n_E = q.energies.n;
+    J.mode = q.mode;
+    J.lo = q.lo;
+    J.dEdx = q.dEdx;
+    J.out = d_out + q.out_off;
+    J.first = first;
+    first += (q.mode ? 10 : 1) * (int64_t)q.energies.n;
+  }
+  hipStream_t s = nullptr;
+  QTRY(hipMemcpyAsync(d_blob, blob, sizeof(double) * (size_t)n_blob, hipMemcpyHostToDevice, s));
+  QTRY(hipMemcpyAsync(d_jobs, hj.data(), sizeof(quad::Job) * (size_t)n_jobs, hipMemcpyHostToDevice, s));
+  QTRY(hipMemsetAsync(d_bad, 0, sizeof(int), s));
+  QTRY(hipMemsetAsync(d_out, 0, sizeof(double) * (size_t)n_out, s));
+  hipLaunchKernelGGL(quad::k_cumint, dim3((unsigned)((3 * n_jobs + 63) / 64)), dim3(64), 0, s, d_jobs, n_jobs);
+  QTRY(hipGetLastError());
+  if (n_int > 0) {
+    hipLaunchKernelGGL(quad::k_track_quad, dim3((unsigned)((n_int + quad::kWavesPerBlock - 1) / quad::kWavesPerBlock)),
+                       dim3(64 * quad::kWavesPerBlock), 0, s, d_jobs, n_jobs, n_int, d_bad);
+    QTRY(hipGetLastError());
+  }
+  int bad = 0;
+  QTRY(hipMemcpyAsync(out, d_out, sizeof(double) * (size_t)n_out, hipMemcpyDeviceToHost, s));
+  QTRY(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s));
+  QTRY(hipStreamSynchronize(s));
+#undef QTRY
+  if (bad) return release(fail(PETITE_ERR_INTERNAL, "petite_dark_quadrature: an integral needs more than 512 geometric panel edges"));
+  return release(PETITE_OK);
+}
+
 int petite_dark_setup(PetiteCtx* c, const PetiteDarkDesc* desc, const double* blob, int64_t n) {
   if (!c || !desc || !blob || n <= 0) return fail(PETITE_ERR_ARG, "petite_dark_setup: bad arguments");
   if (desc->n_active < 0 || desc->n_active > 8) return fail(PETITE_ERR_ARG, "petite_dark_setup: n_active outside 0..8");
@@ -2369,8 +2489,102 @@ int petite_yield_setup(PetiteCtx* c, int32_t n_mass, const double* masses, const
   }
   c->ytab.n_mass = n_mass;
   c->ytab.b = c->yield_blob;
+  c->bnd_on = false;  // a new model brings its own bound-electron inputs (petite_yield_bound)
   if (!c->ytab_dev) HIP_TRY(hipMalloc(&c->ytab_dev, sizeof(dark::YieldTables)));
   HIP_TRY(hipMemcpy(c->ytab_dev, &c->ytab, sizeof(dark::YieldTables), hipMemcpyHostToDevice));
+  return PETITE_OK;
+}
+
+int petite_yield_bound(PetiteCtx* c, const PetiteBoundDesc* d, const double* blob, int64_t n) {
+  if (!c || !d || !blob || n <= 0) return fail(PETITE_ERR_ARG, "petite_yield_bound: bad arguments");
+  if (c->n_mass < 1) return fail(PETITE_ERR_ARG, "petite_yield_bound: set the trained masses first (petite_yield_setup)");
+  auto in_blob = [&](const PetiteQTable& t, int cols) { return t.n >= 1 && t.off >= 0 && t.off + (int64_t)cols * t.n <= n; };
+  for (int q = 0; q < 3; ++q)
+    if (!in_blob(d->loss[q], 2) || !in_blob(d->mfp[q], 2)) return fail(PETITE_ERR_ARG, "petite_yield_bound: bad table");
+  if (!in_blob(d->nodes, 1) || !(d->rel_step > 0.0 && d->rel_step < 0.01) || !(d->dEdx > 0.0) || !(d->Zeff > 0.0) ||
+      !(d->ne_conv > 0.0) || !(d->m_e > 0.0) || !(d->alpha > 0.0))
+    return fail(PETITE_ERR_ARG, "petite_yield_bound: bad parameters");
+  for (int j = 0; j < c->n_mass; ++j)
+    if (!(d->top_ann[j] > 0.0) || !(d->top_comp[j] > 0.0)) return fail(PETITE_ERR_ARG, "petite_yield_bound: bad grid tops");
+  const double* nd = blob + d->nodes.off;
+  for (int i = 1; i < d->nodes.n; ++i)
+    if (!(nd[i] > nd[i - 1])) return fail(PETITE_ERR_ARG, "petite_yield_bound: nodes must ascend strictly");
+  for (void* p : {(void*)c->bnd_blob, (void*)c->bnd_tab, (void*)c->bnd_slots, (void*)c->bnd_jobs, (void*)c->bnd_bad})
+    if (p) (void)hipFree(p);
+  c->bnd_blob = c->bnd_tab = nullptr;
+  c->bnd_slots = nullptr;
+  c->bnd_jobs = nullptr;
+  c->bnd_bad = nullptr;
+  c->bnd_on = false;
+  int64_t n_cum = 0;
+  for (int q = 0; q < 3; ++q) n_cum += d->loss[q].n;
+  const int stride = quad::kBoundTab + d->nodes.n + quad::kBoundGrid;
+  HIP_TRY(hipMalloc(&c->bnd_blob, sizeof(double) * (size_t)(n + n_cum)));
+  HIP_TRY(hipMalloc(&c->bnd_tab, sizeof(double) * (size_t)stride * quad::kBoundMaxSlots));
+  HIP_TRY(hipMalloc(&c->bnd_slots, sizeof(quad::BoundSlot) * quad::kBoundMaxSlots));
+  HIP_TRY(hipMalloc(&c->bnd_jobs, sizeof(quad::Job) * quad::kBoundMaxSlots));
+  HIP_TRY(hipMalloc(&c->bnd_bad, sizeof(int)));
+  HIP_TRY(hipMemset(c->bnd_bad, 0, sizeof(int)));
+  HIP_TRY(hipMemcpy(c->bnd_blob, blob, sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+  // one job per slot; every slot shares the survival channels and their
+  // running integrals (computed once here by the first job)
+  std::vector<quad::Job> jobs(quad::kBoundMaxSlots);
+  int64_t cum_at = n;
+  double* cum[3];
+  for (int q = 0; q < 3; ++q) {
+    cum[q] = c->bnd_blob + cum_at;
+    cum_at += d->loss[q].n;
+  }
+  for (int sl = 0; sl < quad::kBoundMaxSlots; ++sl) {
+    quad::Job& J = jobs[(size_t)sl];
+    double* t = c->bnd_tab + (int64_t)sl * stride;
+    J.lx = t + quad::kBoundGrid;
+    J.ly = t + 2 * quad::kBoundGrid;
+    J.n_ns = quad::kBoundGrid;
+    J.zero_small = 0;  // bound-electron DarkAnn keeps n sigma below 1e-18 (dark_shower.py:311-336)
+    J.n_loss = 3;
+    J.n_mfp = 3;
+    for (int q = 0; q < 3; ++q) {
+      J.loss_x[q] = c->bnd_blob + d->loss[q].off;
+      J.loss_y[q] = c->bnd_blob + d->loss[q].off + d->loss[q].n;
+      J.loss_n[q] = d->loss[q].n;
+      J.grid[q] = J.loss_x[q];
+      J.grid_n[q] = d->loss[q].n;
+      J.cum[q] = cum[q];
+      J.mfp_x[q] = c->bnd_blob + d->mfp[q].off;
+      J.mfp_y[q] = c->bnd_blob + d->mfp[q].off + d->mfp[q].n;
+      J.mfp_n[q] = d->mfp[q].n;
+    }
+    J.n_nodes = 0;  // k_bound_nodes
+    J.nodes = t + quad::kBoundTab;
+    J.E = t;
+    J.n_E = quad::kBoundGrid;
+    J.mode = 0;
+    J.lo = 0.0;  // k_bound_tables
+    J.dEdx = d->dEdx;
+    J.out = t + 3 * quad::kBoundGrid;
+    J.first = (int64_t)sl * quad::kBoundGrid;
+  }
+  HIP_TRY(hipMemcpy(c->bnd_jobs, jobs.data(), sizeof(quad::Job) * jobs.size(), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(quad::k_cumint, dim3(1), dim3(64), 0, nullptr, c->bnd_jobs, 1);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipDeviceSynchronize());
+  c->bnd.Zeff = d->Zeff;
+  c->bnd.ne_conv = d->ne_conv;
+  c->bnd.m_e = d->m_e;
+  c->bnd.alpha = d->alpha;
+  c->bnd.ann_floor = d->ann_floor;
+  c->bnd.comp_floor = d->comp_floor;
+  c->bnd.n_static = d->nodes.n;
+  c->bnd.node_stride = d->nodes.n + quad::kBoundGrid;
+  c->bnd_h = d->rel_step;
+  for (int j = 0; j < c->n_mass; ++j) {
+    c->bnd_top_ann[j] = d->top_ann[j];
+    c->bnd_top_comp[j] = d->top_comp[j];
+  }
+  // the fixed nodes live in the blob at nodes.off
+  c->bnd_on = true;
+  c->bnd_fixed = c->bnd_blob + d->nodes.off;
   return PETITE_OK;
 }
 
@@ -2394,6 +2608,9 @@ int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t
   DevStore S{(float4*)sm->p0, (float4*)sm->r0, (float4*)sm->pf, (float4*)sm->rf, sm->pid, sm->parent, sm->meta,
              sm->shower, sm->sid, sm->capacity};
   const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((n_sm + 255) / 256, (int64_t)c->n_cu * 8));
+  bool use_bound = false;
+  for (int a = 0; a < c->ytab.d[0].n_active; ++a)
+    use_bound |= c->ytab.d[0].active[a] == DARKANN || c->ytab.d[0].active[a] == DARKCOMP;
   // passes over the points in ascending m_V, so that a pass needs few trained masses
   std::vector<int> order(n_points);
   for (int k = 0; k < n_points; ++k) order[k] = k;
@@ -2434,19 +2651,57 @@ int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t
       P.need |= 1u << j;
       if (P.j0 + j + 1 < nm) P.need |= 1u << (j + 1);
     }
+    dark::BoundView Bv{};
+    if (c->bnd_on && use_bound) {
+      // tables at each point's m_V (and m_V (1 +- h) for the derivative),
+      // the grid tops of the trained mass at or below it
+      const int nv = dyield_host ? 3 : 1;
+      quad::BoundPass BP{};
+      BP.n_slots = P.n * nv;
+      for (int k = 0; k < P.n; ++k) {
+        const double m = mV[order[k0 + k]];
+        int j = seg[k0 + k];
+        if (j + 1 < nm && m >= c->masses[j + 1]) ++j;
+        for (int v = 0; v < nv; ++v) {
+          const int sl = k * nv + v;
+          BP.mV[sl] = v == 0 ? m : (v == 1 ? m * (1.0 + c->bnd_h) : m * (1.0 - c->bnd_h));
+          BP.top_ann[sl] = c->bnd_top_ann[j];
+          BP.top_comp[sl] = c->bnd_top_comp[j];
+        }
+        Bv.inv_2h[k] = 1.0 / (2.0 * c->bnd_h * m);
+      }
+      hipLaunchKernelGGL(quad::k_bound_tables, dim3((unsigned)BP.n_slots), dim3(256), 0, s, c->bnd, BP, c->bnd_slots,
+                         c->bnd_tab, c->bnd_jobs);
+      HIP_TRY(hipGetLastError());
+      hipLaunchKernelGGL(quad::k_bound_nodes, dim3(1), dim3(64), 0, s, c->bnd, BP.n_slots, c->bnd_fixed, c->bnd_tab,
+                         c->bnd_jobs);
+      HIP_TRY(hipGetLastError());
+      const int64_t n_int = (int64_t)BP.n_slots * quad::kBoundGrid;
+      hipLaunchKernelGGL(quad::k_track_quad, dim3((unsigned)((n_int + quad::kWavesPerBlock - 1) / quad::kWavesPerBlock)),
+                         dim3(64 * quad::kWavesPerBlock), 0, s, (const quad::Job*)c->bnd_jobs, BP.n_slots, n_int,
+                         c->bnd_bad);
+      HIP_TRY(hipGetLastError());
+      Bv.slot = c->bnd_slots;
+      Bv.tab = c->bnd_tab;
+      Bv.stride = quad::kBoundTab + c->bnd.node_stride;
+      Bv.nv = nv;
+    }
     HIP_TRY(hipMemsetAsync(c->yacc, 0, 2 * dark::kYieldBatch * sizeof(double), s));
     if (n_sm > 0) {
       if (dyield_host)
         hipLaunchKernelGGL(dark::k_dark_yield<true>, dim3((unsigned)blocks), dim3(256), 0, s, c->dev,
-                           (const dark::YieldTables*)c->ytab_dev, S, n_sm, P, c->yacc);
+                           (const dark::YieldTables*)c->ytab_dev, S, n_sm, P, Bv, c->yacc);
       else
         hipLaunchKernelGGL(dark::k_dark_yield<false>, dim3((unsigned)blocks), dim3(256), 0, s, c->dev,
-                           (const dark::YieldTables*)c->ytab_dev, S, n_sm, P, c->yacc);
+                           (const dark::YieldTables*)c->ytab_dev, S, n_sm, P, Bv, c->yacc);
       HIP_TRY(hipGetLastError());
     }
     double h[2 * dark::kYieldBatch];
     HIP_TRY(hipMemcpyAsync(h, c->yacc, sizeof(h), hipMemcpyDeviceToHost, s));
+    int bad = 0;
+    if (Bv.slot) HIP_TRY(hipMemcpyAsync(&bad, c->bnd_bad, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    if (bad) return fail(PETITE_ERR_INTERNAL, "petite_dark_yield: a bound-electron weight needs more than 512 panel edges");
     for (int k = 0; k < P.n; ++k) {
       yield_host[order[k0 + k]] = h[2 * k];
       if (dyield_host) dyield_host[order[k0 + k]] = h[2 * k + 1];

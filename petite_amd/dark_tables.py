@@ -29,7 +29,14 @@ from .constants import (TWO_BODY_CODE, GeVsqcm2, alpha_em, cmtom, m_electron, m_
 from .tables import DATA_DIR, MaterialTables, _load, geometric, trained_grid, trained_pack
 
 DARK_PROCESSES = ["DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"]
-_GL_X, _GL_W = np.polynomial.legendre.leggauss(8)
+# survival channels per species (shower.py:322-354) and the summation order of
+# the mean free path (get_mfp with _NSigmaElectron/Positron/Muon, shower.py:355-389)
+SURVIVAL = {11: ("Brem", "Moller"), -11: ("Brem", "Ann", "Bhabha"), 13: ("MuonBrem", "MuonE"),
+            -13: ("MuonBrem", "MuonE")}
+MFP_ORDER = {11: ("Brem", "Moller"), -11: ("Brem", "Bhabha", "Ann"), 13: ("MuonBrem", "MuonE"),
+             -13: ("MuonBrem", "MuonE")}
+WEIGHT_FAMILIES = (("brem_elec", "DarkBrem", 11), ("brem_positron", "DarkBrem", -11),
+                   ("muon_brem", "DarkMuonBrem", 13), ("annihilation", "DarkAnn", -11))
 
 
 class _Packs:
@@ -239,16 +246,77 @@ def _lin0(xg, yg, x):
     return np.where((x < xg[0]) | (x > xg[-1]), 0.0, np.interp(x, xg, yg))
 
 
-def _cumint_pl(xg, yg, nodes):
-    """Exact integral from nodes[0] to each node of the piecewise-linear
-    interpolant (xg, yg), zero outside [xg[0], xg[-1]]."""
-    nodes = np.asarray(nodes, dtype=np.float64)
-    pts = np.union1d(nodes, xg[(xg > nodes[0]) & (xg < nodes[-1])])
-    f = _lin0(xg, yg, pts)
-    seg = 0.5 * (f[1:] + f[:-1]) * np.diff(pts)
-    # a jump of the fill at the ends of xg would need the one-sided limits
-    cum = np.concatenate([[0.0], np.cumsum(seg)])
-    return np.interp(nodes, pts, cum)
+def quadrature_loss_tables(sm):
+    """{channel: (x, y, grid)}: the n sigma table of each SM interaction
+    integral and the grid its running integral is kept on -- its own, except
+    MuonBrem, whose integral is the one of MuonE on the MuonBrem grid
+    (shower.py:316)."""
+    out = {P: (sm.nsigma_tab[P][0], sm.nsigma_tab[P][1], sm.nsigma_tab[P][0])
+           for P in ("Brem", "Ann", "Moller", "Bhabha", "MuonE")}
+    out["MuonBrem"] = (sm.nsigma_tab["MuonE"][0], sm.nsigma_tab["MuonE"][1], sm.nsigma_tab["MuonBrem"][0])
+    return out
+
+
+def dark_quadrature(sm, specs):
+    """Weights (mode 0) and drate rows (mode 1) of the given jobs on the
+    device (petite_dark_quadrature).  Each spec: lx, ly (log-log n sigma of the
+    process), zero_small, pid (survival and mean-free-path channels), nodes,
+    energies, lo, mode.  Returns per spec the (n,) weights or (n, 10, 2) rows."""
+    from . import _lib
+    parts, n = [], 0
+    cache = {}
+
+    def put(*arrays):
+        nonlocal n
+        off = n
+        for a in arrays:
+            a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+            parts.append(a)
+            n += a.size
+        return off
+
+    def table(key, x, y=None):
+        if key not in cache:
+            t = _lib.QTable()
+            t.off = put(x) if y is None else put(x, y)
+            t.n = len(x)
+            cache[key] = t
+        return cache[key]
+
+    loss = quadrature_loss_tables(sm)
+    jobs = (_lib.DarkQuadJob * len(specs))()
+    out_n = 0
+    for i, sp in enumerate(specs):
+        J = jobs[i]
+        pid = int(sp["pid"])
+        J.ns = table(("ns", i), sp["lx"], sp["ly"])
+        J.n_loss = len(SURVIVAL[pid])
+        for c, P in enumerate(SURVIVAL[pid]):
+            x, y, g = loss[P]
+            J.loss[c] = table(("loss", P), x, y)
+            J.grid[c] = table(("grid", P), g)
+        J.n_mfp = len(MFP_ORDER[pid])
+        for c, P in enumerate(MFP_ORDER[pid]):
+            J.mfp[c] = table(("mfp", P), *sm.nsigma_tab[P])
+        J.nodes = table(("nodes", i), sp["nodes"])
+        J.energies = table(("E", i), sp["energies"])
+        J.mode = int(sp["mode"])
+        J.zero_small = 1 if sp["zero_small"] else 0
+        J.lo = float(sp["lo"])
+        J.dEdx = float(sm.dEdx)
+        J.out_off = out_n
+        out_n += len(sp["energies"]) * (20 if J.mode else 1)
+    blob = np.concatenate(parts) if parts else np.zeros(1)
+    out = np.zeros(max(out_n, 1))
+    lib = _lib.load()
+    _lib.check(lib.petite_dark_quadrature(jobs, len(specs), blob.ctypes.data, blob.size, out.ctypes.data, out.size),
+               "petite_dark_quadrature")
+    res = []
+    for i, sp in enumerate(specs):
+        m = len(sp["energies"])
+        o = jobs[i].out_off
+        res.append(out[o:o + m].copy() if sp["mode"] == 0 else out[o:o + 20 * m].reshape(m, 10, 2).copy())
+    return res
 
 
 class DarkTables:
@@ -304,8 +372,9 @@ class DarkTables:
                    "DarkAnn": LogLog(Ea, ne * GeVsqcm2 * self.xsec["DarkAnn"][:, 1]),
                    "DarkComp": LogLog(Ec, ne * GeVsqcm2 * self.xsec["DarkComp"][:, 1]),
                    "DarkMuonBrem": LogLog(dmb[:, 0], nZ * GeVsqcm2 * dmb[:, 1])}
-        self._build_survival()
-        self._build_weights(z, key)
+        self._z, self._key = z, key
+        self.from_cache = f"weights__brem_elec_weights__{key}__{self.material}" in z.files
+        self._W = self._D = None
         self.grids = {}
         self.map_source = {}
         fallback = None
@@ -340,138 +409,62 @@ class DarkTables:
         # n*sigma of the dark processes for the device (log-log -> tabulate as given)
         self.nsigma_tab = {}
 
-    # -- survival factors (shower.py:298-354) ---------------------------------
-    def _build_survival(self):
-        sm = self.sm
-        ii = {}
-        for P in ("Brem", "PairProd", "Ann", "Comp", "Moller", "Bhabha", "MuonE"):
-            xg, yg = sm.nsigma_tab[P]
-            ii[P] = (xg, _cumint_pl(xg, yg, xg))
-        # the reference integrates n sigma(MuonE) on the MuonBrem grid (shower.py:316)
-        mbx = sm.nsigma_tab["MuonBrem"][0]
-        mex, mey = sm.nsigma_tab["MuonE"]
-        ii["MuonBrem"] = (mbx, _cumint_pl(mex, mey, mbx))
-        self.ii = ii
-        self.dEdxT = sm.dEdx  # GeV/m
-
-    def _II(self, P, E):
-        xg, yg = self.ii[P]
-        return _lin0(xg, yg, E)
-
-    def survival(self, pid, E, Ei):
-        """_electron/_positron/_muon_exponential_factor (shower.py:322-354), vectorised in E."""
-        chans = {11: ("Brem", "Moller"), -11: ("Brem", "Ann", "Bhabha"), 13: ("MuonBrem", "MuonE"),
-                 -13: ("MuonBrem", "MuonE")}[pid]
-        E = np.asarray(E, dtype=np.float64)
-        d = sum(self._II(P, Ei) - self._II(P, E) for P in chans)
-        with np.errstate(over="ignore"):
-            out = np.exp(-d / self.dEdxT / cmtom)
-        return np.where((d < 0.0) | (E > Ei), 0.0, out)
-
-    def mfp(self, pid, E):
-        sm = self.sm
-        chans = {11: ("Brem", "Moller"), -11: ("Brem", "Ann", "Bhabha"), 13: ("MuonE", "MuonBrem"),
-                 -13: ("MuonE", "MuonBrem")}[pid]
-        ns = float(sum(sm.nsigma(P, E) for P in chans))
-        return 1.0e12 if ns <= 0.0 else cmtom / ns
-
     # -- weights and rates (dark_shower.py:311-593) -----------------------------
-    def _integrand(self, proc, pid, E, Ei):
-        k = self.sm.dEdx * cmtom  # GeV/cm
-        n = self.ns[proc](E)
-        if proc != "DarkAnn":
-            n = np.where(n < 1.0e-18, 0.0, n)
-        return n / k * self.survival(pid, E, Ei)
+    # Track-length integrals over the survival factor of the SM interaction
+    # integrals (shower.py:298-354), computed on the device by
+    # petite_dark_quadrature on first use.
+    @property
+    def W(self):
+        if self._W is None:
+            self._build_weights()
+        return self._W
 
-    def _panels(self, lo, hi, Ei, scale, nodes):
-        """Panel edges on [lo, hi]: geometric in the distance below Ei, starting
-        at scale/8, plus every table node inside."""
-        if hi <= lo:
-            return np.array([lo, hi])
-        edges = [hi]
-        d = scale / 8.0
-        while Ei - d > lo:
-            if Ei - d < hi:
-                edges.append(Ei - d)
-            d *= 1.35
-        edges.append(lo)
-        inner = nodes[(nodes > lo) & (nodes < hi)]
-        return np.unique(np.concatenate([np.array(edges), inner]))
+    @property
+    def D(self):
+        if self._D is None:
+            self._build_weights()
+        return self._D
 
-    def integrate(self, proc, pid, lo, hi, Ei, nodes):
-        if hi <= lo:
-            return 0.0
-        scale = max(10.0 * self.mfp(pid, Ei) * self.dEdxT, 1e-9)
-        e = self._panels(lo, hi, Ei, scale / 10.0, nodes)
-        a, b = e[:-1], e[1:]
-        h = 0.5 * (b - a)
-        c = 0.5 * (b + a)
-        x = c[:, None] + h[:, None] * _GL_X[None, :]
-        f = self._integrand(proc, pid, x.ravel(), Ei).reshape(x.shape)
-        return float(np.sum(h * (f @ _GL_W)))
+    def _energies(self, proc):
+        return {"DarkAnn": self.xsec["DarkAnn"][:, 0], "DarkMuonBrem": self.xsec["DarkMuonBrem"][:, 0]}.get(
+            proc, self.xsec["DarkBrem"][:, 0])
 
     def _nodes(self, proc, pid):
-        sm = self.sm
-        chans = {11: ("Brem", "Moller"), -11: ("Brem", "Ann", "Bhabha"), 13: ("MuonBrem", "MuonE"),
-                 -13: ("MuonBrem", "MuonE")}[pid]
-        pts = [self.ii[P][0] for P in chans]
-        lx = self.ns[proc].lx
-        pts.append(10.0 ** lx)
+        """Panel nodes: the grids of the species' survival integrals and the
+        nodes of the process's log-log n sigma table."""
+        loss = quadrature_loss_tables(self.sm)
+        pts = [loss[P][2] for P in SURVIVAL[pid]]
+        pts.append(10.0 ** self.ns[proc].lx)
         return np.unique(np.concatenate(pts))
 
-    def weight_array(self, proc, pid):
-        """Track-length weights on the process's own energy grid (construct_*_weight_array)."""
-        if proc == "DarkAnn":
-            Ei_all = self.xsec["DarkAnn"][:, 0]
-        elif proc == "DarkMuonBrem":
-            Ei_all = self.xsec["DarkMuonBrem"][:, 0]
-        else:
-            Ei_all = self.xsec["DarkBrem"][:, 0]
-        lo = Ei_all[0]
-        nodes = self._nodes(proc, pid)
-        W = np.array([self.integrate(proc, pid, lo, Ei, Ei, nodes) for Ei in Ei_all])
-        return np.column_stack([Ei_all, W])
-
-    def drate(self, proc, pid, Ei, lo_table):
-        """_d_rate_d_E_*: 10 equal bins over [max(Ei - 10 mfp dE/dx, E_table0), Ei]."""
-        lo = max(Ei - 10.0 * self.mfp(pid, Ei) * self.dEdxT, lo_table)
-        edges = np.linspace(lo, Ei, 11)
-        nodes = self._nodes(proc, pid)
-        w = np.array([self.integrate(proc, pid, edges[i], edges[i + 1], Ei, nodes) for i in range(10)])
-        return np.column_stack([0.5 * (edges[1:] + edges[:-1]), w])
-
-    def _build_weights(self, z, key):
-        mat = self.material
-        cached = f"weights__brem_elec_weights__{key}__{mat}" in z.files
-        self.from_cache = cached
-        W = {}
-        if cached:
-            W["brem_elec"] = np.array(z[f"weights__brem_elec_weights__{key}__{mat}"])
-            W["brem_positron"] = np.array(z[f"weights__brem_positron_weights__{key}__{mat}"])
-            W["muon_brem"] = np.array(z[f"weights__muon_brem_weights__{key}__{mat}"])
-        else:
-            W["brem_elec"] = self.weight_array("DarkBrem", 11)
-            W["brem_positron"] = self.weight_array("DarkBrem", -11)
-            W["muon_brem"] = self.weight_array("DarkMuonBrem", 13)
-        W["annihilation"] = self.weight_array("DarkAnn", -11)  # bound: always recomputed (dark_shower.py:450-451)
-        self.W = W
-        D = {}
-        dbE = self.xsec["DarkBrem"][:, 0]
-        dmE = self.xsec["DarkMuonBrem"][:, 0]
-        daE = self.xsec["DarkAnn"][:, 0]
-        for name, proc, pid, Es, lo in (("brem_elec", "DarkBrem", 11, dbE, dbE[0]),
-                                         ("brem_positron", "DarkBrem", -11, dbE, dbE[0]),
-                                         ("muon_brem", "DarkMuonBrem", 13, dmE, dmE[0])):
-            ck = f"drate__{name}_drate__{key}__{mat}"
-            if ck in z.files:
-                arr = np.array(z[ck])
-                cnt = np.array(z[f"dratecnt__{name}_drate__{key}__{mat}"])
-                D[name] = (np.array(z[f"drateE__{name}_drate__{key}__{mat}"]), arr[:, :10, :], cnt)
+    def _build_weights(self):
+        z, key, mat = self._z, self._key, self.material
+        cached = self.from_cache
+        W, D, jobs = {}, {}, []
+        for name, proc, pid in WEIGHT_FAMILIES:
+            Es = self._energies(proc)
+            ll = self.ns[proc]
+            spec = dict(lx=ll.lx, ly=ll.ly, zero_small=proc != "DarkAnn", pid=pid, nodes=self._nodes(proc, pid),
+                        energies=Es, lo=Es[0])
+            if cached and name != "annihilation":
+                W[name] = np.array(z[f"weights__{name}_weights__{key}__{mat}"])
             else:
-                D[name] = (Es, np.stack([self.drate(proc, pid, Ei, lo) for Ei in Es]), np.full(len(Es), 10))
-        D["annihilation"] = (daE, np.stack([self.drate("DarkAnn", -11, Ei, daE[0]) for Ei in daE]),
-                             np.full(len(daE), 10))
-        self.D = D
+                jobs.append((("W", name), dict(spec, mode=0)))
+            ck = f"drate__{name}_drate__{key}__{mat}"
+            if name != "annihilation" and ck in z.files:
+                arr = np.array(z[ck])
+                D[name] = (np.array(z[f"drateE__{name}_drate__{key}__{mat}"]), arr[:, :10, :],
+                           np.array(z[f"dratecnt__{name}_drate__{key}__{mat}"]))
+            else:
+                jobs.append((("D", name), dict(spec, mode=1)))
+        res = dark_quadrature(self.sm, [j for _, j in jobs])
+        for ((kind, name), spec), out in zip(jobs, res):
+            Es = spec["energies"]
+            if kind == "W":
+                W[name] = np.column_stack([Es, out])
+            else:
+                D[name] = (Es, out, np.full(len(Es), 10))
+        self._W, self._D = W, D
 
     def bsm_weight(self, pid, E0, process):
         """GetBSMWeights (dark_shower.py:595-647) for arrays of (pid, E0)."""

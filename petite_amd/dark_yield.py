@@ -7,7 +7,12 @@ Y(m_V) = sum_i w_i sum_P W_P(E0_i; m_V) with W_P = GetBSMWeights (595-647),
 interpolates each particle's weight between the trained masses m_j < m_{j+1}:
 W_j^(1-t) W_{j+1}^t (t = ln(m_V/m_j) / ln(m_{j+1}/m_j)) where both are positive,
 (1-t) W_j + t W_{j+1} otherwise; at a trained mass it equals the weight sum of
-DarkShower.generate_dark_shower's entries.  The trained masses are the
+DarkShower.generate_dark_shower's entries.  The bound-electron processes
+(DarkAnn, DarkComp) are not interpolated: their weights are a resonance and a
+threshold that move with m_V, so the device builds their tables at each
+requested m_V (closed-form cross sections, DarkAnn track-length weights;
+petite_yield_bound), and their dY/dm_V is the central difference of tables
+at m_V (1 +- 1e-5).  The trained masses are the
 shipped ones and those of the generated mass pack
 (petite_amd.tablegen.dark_masses); a mass outside [lightest, heaviest] raises
 PetiteError (petite_dark_yield rejects it: nothing to interpolate).  One float64 pass of the
@@ -21,7 +26,8 @@ import torch
 
 from . import _lib
 from .constants import all_process_code, dark_process_codes
-from .dark_tables import DarkTables, build_dark_blob
+from .constants import GeVsqcm2, alpha_em, m_electron
+from .dark_tables import MFP_ORDER, SURVIVAL, DarkTables, build_dark_blob, quadrature_loss_tables
 from .device import DeviceContext, _stream_handle
 from .tables import MaterialTables
 
@@ -62,8 +68,41 @@ class DarkYield:
             n += blob.size
         self._blob = np.concatenate(parts)
         self._descs = descs
+        self._bound, self._bound_blob = self._bound_inputs(Zeff)
         self.ctx = ctx if ctx is not None else DeviceContext(tables, min_energy)
         self._setup()
+
+    def _bound_inputs(self, Zeff, rel_step=1e-5):
+        """Inputs of petite_yield_bound: the positron survival and mean-free-path
+        tables of the material and the grid tops of each trained mass."""
+        sm = self.tables
+        d = _lib.BoundDesc()
+        d.Zeff, d.ne_conv, d.m_e, d.alpha = float(Zeff), float(sm.ne * GeVsqcm2), m_electron, alpha_em
+        d.ann_floor, d.comp_floor = float(sm.min_calc[-11]), float(sm.min_calc[22])
+        d.rel_step, d.dEdx = rel_step, float(sm.dEdx)
+        for j, m in enumerate(self.masses):
+            d.top_ann[j] = float(self.dark[m].xsec["DarkBrem"][-1, 0])
+            d.top_comp[j] = float(self.dark[m].xsec["DarkComp_free"][-1, 0])
+        parts, n = [], 0
+
+        def put(*arrays):
+            nonlocal n
+            off = n
+            for a in arrays:
+                parts.append(np.ascontiguousarray(a, dtype=np.float64).ravel())
+                n += parts[-1].size
+            return off
+
+        loss = quadrature_loss_tables(sm)
+        for c, P in enumerate(SURVIVAL[-11]):
+            x, y, _ = loss[P]
+            d.loss[c].off, d.loss[c].n = put(x, y), len(x)
+        for c, P in enumerate(MFP_ORDER[-11]):
+            x, y = sm.nsigma_tab[P]
+            d.mfp[c].off, d.mfp[c].n = put(x, y), len(x)
+        nodes = np.unique(np.concatenate([loss[P][2] for P in SURVIVAL[-11]]))
+        d.nodes.off, d.nodes.n = put(nodes), len(nodes)
+        return d, np.concatenate(parts)
 
     def _setup(self):
         """Load this model's tables into the context (a context holds one
@@ -72,6 +111,9 @@ class DarkYield:
         _lib.check(self.ctx.lib.petite_yield_setup(self.ctx.ctx, len(self.masses), m.ctypes.data,
                                                    C.cast(self._descs, C.c_void_p), self._blob.ctypes.data,
                                                    self._blob.size), "petite_yield_setup")
+        if any(c in self.active_codes for c in (all_process_code["DarkAnn"], all_process_code["DarkComp"])):
+            _lib.check(self.ctx.lib.petite_yield_bound(self.ctx.ctx, C.byref(self._bound), self._bound_blob.ctypes.data,
+                                                       self._bound_blob.size), "petite_yield_bound")
         self.ctx._yield_owner = self
 
     def evaluate(self, record, mV, grad=True):

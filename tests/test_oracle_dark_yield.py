@@ -12,8 +12,7 @@ import pytest
 
 from oracle.dark import OracleDark, OracleDarkSampler, sampler_weights, yield_and_grad
 from oracle.tables import SMTables
-from petite_amd.dark_tables import DarkTables
-from petite_amd.tables import MaterialTables
+from oracle.dark_panels import Panels
 
 MASSES = [0.01, 0.03, 0.1]
 PROCS = ["DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"]
@@ -21,11 +20,11 @@ PROCS = ["DarkBrem", "DarkAnn", "DarkComp", "DarkMuonBrem"]
 
 @pytest.fixture(scope="module")
 def setup():
-    mt, T = MaterialTables("graphite"), SMTables("graphite")
+    T = SMTables("graphite")
     samplers = []
     for m in MASSES:
-        dt = DarkTables(mt, m)
-        samplers.append(OracleDarkSampler(OracleDark(T, m), dt.W, dt.D, None))
+        od = OracleDark(T, m)
+        samplers.append(OracleDarkSampler(od, *Panels(od).tables(), None))
     rng = np.random.default_rng(8)
     n = 400
     pid = rng.choice([11, -11, 22, 13], size=n)
@@ -75,3 +74,26 @@ def test_geometric_interpolation_exact_for_power_law(monkeypatch):
                               [0.017, 0.06])
     np.testing.assert_allclose(Y, E0.sum() / np.array([0.017, 0.06]) ** 2, rtol=1e-12)
     np.testing.assert_allclose(dY, -2 * E0.sum() / np.array([0.017, 0.06]) ** 3, rtol=1e-12)
+
+
+def test_bound_yield_at_trained_mass_equals_weight_sum(setup):
+    # tables built at a trained mass are that mass's own tables
+    from oracle.dark import bound_yield
+    samplers, pid, E0, w = setup
+    T = samplers[0].T
+    for j, m in enumerate(MASSES):
+        Yb, _ = bound_yield(T, ["DarkAnn", "DarkComp"], pid, E0, w, [m])
+        want = sum(np.sum(sampler_weights(samplers[j], pid, E0, P) * w) for P in ("DarkAnn", "DarkComp"))
+        assert want > 0 and abs(Yb[0] / want - 1) < 1e-12, (m, Yb[0], want)
+
+
+def test_bound_yield_derivative_is_a_consistent_central_difference(setup):
+    from oracle.dark import bound_yield
+    samplers, pid, E0, w = setup
+    T = samplers[0].T
+    m = 0.013
+    Y, dY = bound_yield(T, ["DarkAnn", "DarkComp"], pid, E0, w, [m])
+    h = 1e-4
+    Yp, _ = bound_yield(T, ["DarkAnn", "DarkComp"], pid, E0, w, [m * (1 + h)])
+    Ym, _ = bound_yield(T, ["DarkAnn", "DarkComp"], pid, E0, w, [m * (1 - h)])
+    assert abs(dY[0] / ((Yp[0] - Ym[0]) / (2 * h * m)) - 1) < 1e-5
