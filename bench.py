@@ -9,7 +9,11 @@ its descendants to the end (PETITE's generate_shower for each primary).
 
 Prints one JSON line: particle-steps/s (value), showers/s, the roofline of
 the dominant kernel (from HIP events on the run's stream) and a CPU baseline
-(the float64 oracle, one core, on a bounded sample of the same primaries).
+(the float64 oracle on a stratified sample of the same primaries: one core,
+and all cores of a process pool), per-process draw statistics with the
+max_F violations, and the reference's own CPU case beside the GPU under
+"c1_graphite_10GeV" (1,000 showers of 10 GeV e- in graphite; the oracle per
+shower against the published 2.59 s).
 The dark-yield scan follows under "dark_yield_scan" (a fixed record of 50,000
 e- showers in lead, yield and d/dm_V at 32 masses: gradient evaluations/s and
 the gradient/forward cost ratio) with its CPU baseline; --dark-forward adds
@@ -57,26 +61,174 @@ def make_primaries(n=N_PRIMARIES, seed=SEED):
     return batch_primaries(n, seed)
 
 
-def cpu_baseline(prim, maxf, budget_s=20.0):
-    """Oracle (float64, scalar Python, per-candidate draws) on the first primaries."""
+def tuning_report(ctx):
+    """Where the run's launch sizes came from, and the autotuner's table entry
+    for this device (its best time against the built-in defaults)."""
+    from petite_amd import autotune
+    out = {"source": ctx.tuning_source, "point": ctx.get_tuning()}
+    try:
+        import json as _json
+        with open(autotune.TABLE) as f:
+            ent = _json.load(f).get(autotune.device_key())
+        if ent:
+            out.update({"table_ms": ent["ms"], "table_default_ms": ent["default_ms"], "table_workload": ent["workload"]})
+    except (OSError, ValueError):
+        pass
+    return out
+
+
+def _oracle_shower(args):
+    """One oracle shower in a worker process: (particle-steps, seconds)."""
+    material, cutoff, maxf, pid, p4, sid, mass = args
     from oracle.shower import OracleShower
     from oracle.tables import SMTables
-    T = SMTables(MATERIAL)
-    osh = OracleShower(T, CUTOFF, seed=SEED, maxf=maxf, vectorised_draw=False)
+    osh = OracleShower(SMTables(material), cutoff, seed=SEED, maxf=maxf, vectorised_draw=False)
     t0 = time.perf_counter()
-    n = 0
-    steps0 = osh.stats.steps
-    while n < len(prim) and time.perf_counter() - t0 < budget_s:
-        r = prim[n]
-        osh.generate_shower(int(r["pid"]), [r["E"], r["px"], r["py"], r["pz"]], sid=int(r["sid"]), shower=n,
-                            mass=float(r["mass"]))
-        n += 1
-    dt = time.perf_counter() - t0
-    steps = osh.stats.steps - steps0
-    return {"value": steps / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "showers_per_s": n / dt,
-            "sample": f"first {n} of the {len(prim)} primaries, {dt:.1f} s on one core "
-                      f"(oracle float64 restatement, per-candidate accept/reject)"}
+    osh.generate_shower(pid, p4, sid=sid, shower=0, mass=mass)
+    return osh.stats.steps, time.perf_counter() - t0
+
+
+def _oracle_pool(jobs, workers):
+    """Run oracle showers on `workers` processes (spawned: they start a fresh
+    interpreter with one BLAS thread each, never a fork of this GPU process).
+    Returns (per-job results, wall seconds)."""
+    import multiprocessing as mp
+    env = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    for k in env:
+        os.environ[k] = "1"
+    try:
+        t0 = time.perf_counter()
+        with mp.get_context("spawn").Pool(workers) as pool:
+            res = pool.map(_oracle_shower, jobs, chunksize=1)
+        wall = time.perf_counter() - t0
+    finally:
+        for k, v in env.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    return res, wall
+
+
+def _workers():
+    n = os.cpu_count() or 1
+    return max(1, min(16, n))  # the GPU box gives one GPU's share of 16 cores
+
+
+def cpu_baseline(prim, maxf, steps_per_shower, strata=20):
+    """Oracle (float64, scalar Python, per-candidate draws like draw_sample) on
+    a stratified sample of the workload: the first primary of the list in each
+    of `strata` equal log-energy strata of [0.1, 100] GeV.  One core: summed
+    particle-steps over summed per-shower CPU time; all cores: the same
+    showers on a process pool, over its wall time.  Showers/s: the one-core
+    rate over the workload's mean particle-steps per shower (the GPU's showers
+    are the oracle's, particle for particle)."""
+    edges = np.geomspace(0.1, 100.0, strata + 1)
+    picks = []
+    for lo, hi in zip(edges[:-1], edges[1:]):
+        idx = np.nonzero((prim["E"] >= lo) & (prim["E"] < hi))[0]
+        if len(idx):
+            picks.append(int(idx[0]))
+    picks.sort(key=lambda i: -float(prim["E"][i]))  # the longest first
+    jobs = [(MATERIAL, CUTOFF, maxf, int(prim["pid"][i]), [float(prim["E"][i]), float(prim["px"][i]),
+             float(prim["py"][i]), float(prim["pz"][i])], int(prim["sid"][i]), float(prim["mass"][i])) for i in picks]
+    workers = _workers()
+    res, wall = _oracle_pool(jobs, workers)
+    steps = sum(r[0] for r in res)
+    cpu_s = sum(r[1] for r in res)
+    one = steps / cpu_s
+    return {"value": one, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "showers_per_s": one / steps_per_shower,
+            "all_cores": {"cores": workers, "value": steps / wall, "unit": "particle-steps/s",
+                          "showers_per_s": steps / wall / steps_per_shower, "wall_s": wall},
+            "sample": f"{len(picks)} primaries, the first of the workload list in each of {strata} equal log-energy "
+                      f"strata of [0.1, 100] GeV ({steps} particle-steps, {cpu_s:.1f} s of CPU); showers/s = "
+                      f"particle-steps/s over the workload's {steps_per_shower:.0f} particle-steps per shower; "
+                      f"oracle float64 restatement, per-candidate accept/reject"}
+
+
+PROC_NAMES = ("Brem", "Ann", "PairProd", "Comp", "Moller", "Bhabha", "MuonE", "MuonBrem")
+
+
+def bench_c5(n=10_000, reps=2):
+    """The 400 GeV workload: n primaries at 400 GeV along +z, half muons and
+    half electrons, lead, cutoff 0.100 GeV, the 400 GeV table set with its
+    GPU-trained Brem/PairProd/MuonBrem maps (best of `reps` runs)."""
+    import torch
+
+    from petite_amd import Shower
+    from petite_amd.batch import PRIMARY_DTYPE
+    from petite_amd.constants import mass_dict
+    prim = np.zeros(n, dtype=PRIMARY_DTYPE)
+    pid = np.where(np.arange(n) < n // 2, 13, 11).astype(np.int32)
+    m = np.array([mass_dict[int(q)] for q in pid])
+    prim["E"], prim["pz"], prim["mass"], prim["weight"], prim["pid"] = 400.0, np.sqrt(400.0 ** 2 - m * m), m, 1.0, pid
+    prim["shower"] = np.arange(n)
+    prim["sid"] = np.arange(n)
+    sh = Shower("400GeV", "lead", 0.100, seed=SEED)
+    b = sh.generate_showers(prim, seed=SEED)
+    best = 1e9
+    for _ in range(reps):
+        del b
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b = sh.generate_showers(prim, seed=SEED)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    st = b.stats
+    return {"workload": f"{n} primaries at 400 GeV along +z (half mu-, half e-), lead, cutoff 0.100 GeV, 400 GeV tables "
+                        f"with GPU-trained maps",
+            "showers_per_s": n / best, "particle_steps_per_s": st["n_steps"] / best, "ms": best * 1e3,
+            "particles": st["n_particles"], "maps": dict(sh._tables.map_source),
+            "draws_by_process": {name: {"draws": d, "trials_per_draw": t / d, "clipped_fraction": v / d}
+                                 for name, d, t, v in zip(PROC_NAMES, st["n_draws_proc"], st["n_trials_proc"],
+                                                          st["n_viol_proc"]) if d > 0}}
+
+
+def bench_c1(cpu=True, n=1000, reps=3):
+    """The reference's own CPU case: 10 GeV e- along +z in graphite, cutoff
+    0.010 GeV, one shower per seed 0..n-1 (Philox stream id = seed).  GPU:
+    all n showers in one batched run (best of `reps`); CPU: the oracle on
+    one shower per worker, against the published 2.59 s per shower (the
+    reference under cProfile on the author's machine)."""
+    import torch
+
+    from petite_amd import Shower
+    from petite_amd.batch import PRIMARY_DTYPE
+    me = 0.51099895e-3
+    prim = np.zeros(n, dtype=PRIMARY_DTYPE)
+    prim["E"], prim["pz"], prim["mass"], prim["weight"], prim["pid"] = 10.0, math.sqrt(100.0 - me * me), me, 1.0, 11
+    prim["shower"] = np.arange(n)
+    prim["sid"] = np.arange(n)
+    sh = Shower("100GeV", "graphite", CUTOFF, seed=SEED)
+    b = sh.generate_showers(prim, seed=SEED)
+    best = 1e9
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b = sh.generate_showers(prim, seed=SEED)
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    st = b.stats
+    out = {"workload": f"{n} showers of 10 GeV e- along +z, graphite, cutoff {CUTOFF} GeV, 100 GeV tables, "
+                       f"stream ids 0..{n - 1}",
+           "showers_per_s": n / best, "particle_steps_per_s": st["n_steps"] / best,
+           "particles_per_shower": st["n_particles"] / n, "ms": best * 1e3}
+    if cpu:
+        ctx = sh.device_context()
+        maxf = {(P, ie): float(v) for P, arr in ctx.maxf.items() for ie, v in enumerate(arr)}
+        workers = _workers()
+        jobs = [("graphite", CUTOFF, maxf, 11, [10.0, 0.0, 0.0, math.sqrt(100.0 - me * me)], i, me)
+                for i in range(workers)]
+        res, wall = _oracle_pool(jobs, workers)
+        s_per = sum(r[1] for r in res) / len(res)
+        out["cpu_baseline"] = {"s_per_shower": s_per, "showers_per_s": 1.0 / s_per, "cores": 1, "kind": "port",
+                               "all_cores": {"cores": workers, "showers_per_s": len(res) / wall},
+                               "published_reference_s_per_shower": 2.59,
+                               "sample": f"stream ids 0..{workers - 1}, one shower per worker process, oracle "
+                                         f"float64 restatement with per-candidate accept/reject"}
+        out["gpu_over_cpu_showers"] = out["showers_per_s"] / out["cpu_baseline"]["showers_per_s"]
+    return out
 
 
 def copy_calibration():
@@ -101,8 +253,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--primaries", type=int, default=N_PRIMARIES)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-dark", action="store_true", help="skip the dark-yield workload")
+    ap.add_argument("--showers-only", action="store_true",
+                    help="only the value workload (no C1, detector, map-training, meson or dark legs): for profiling")
     ap.add_argument("--dark-forward", action="store_true",
                     help="also run the dark-vector workload (20,000 e- at 10 GeV, m_V = 1 MeV)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -172,18 +325,28 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         maxf = {(P, ie): float(v) for P, arr in ctx.maxf.items() for ie, v in enumerate(arr)}
-        cpu = cpu_baseline(prim, maxf, args.cpu_budget)
+        cpu = cpu_baseline(prim, maxf, totals["steps"] / K / len(prim))
+        print(f"cpu baseline: {cpu['value']:.0f} particle-steps/s on one core", file=sys.stderr, flush=True)
 
-    det_out = bench_detector(b, cpu=not args.no_cpu_baseline)
-    print(f"detector: {det_out['records']} records, {det_out['ms']:.3f} ms", file=sys.stderr, flush=True)
+    c1_out = c5_out = det_out = train_out = meson_out = None
+    if not args.showers_only:
+        c1_out = bench_c1(cpu=not args.no_cpu_baseline)
+        print(f"c1: {c1_out['showers_per_s']:.0f} showers/s", file=sys.stderr, flush=True)
+        c5_out = bench_c5()
+        print(f"c5: {c5_out['showers_per_s']:.0f} showers/s", file=sys.stderr, flush=True)
+        det_out = bench_detector(b, cpu=not args.no_cpu_baseline)
+        print(f"detector: {det_out['records']} records, {det_out['ms']:.3f} ms", file=sys.stderr, flush=True)
     del b
-    train_out = bench_map_training(cpu=not args.no_cpu_baseline)
-    print(f"map training: {train_out['samples']} samples, {train_out['s_total']:.2f} s", file=sys.stderr, flush=True)
-
-    meson_out = bench_meson_beam(cpu=not args.no_cpu_baseline)
-    print(f"meson beam: {meson_out['records']} records, {meson_out['ms']:.3f} ms", file=sys.stderr, flush=True)
+    if not args.showers_only:
+        train_out = bench_map_training(cpu=not args.no_cpu_baseline)
+        print(f"map training: {train_out['samples']} samples, {train_out['s_total']:.2f} s", file=sys.stderr,
+              flush=True)
+        meson_out = bench_meson_beam(cpu=not args.no_cpu_baseline)
+        print(f"meson beam: {meson_out['records']} records, {meson_out['ms']:.3f} ms", file=sys.stderr, flush=True)
 
     dark_out = yield_out = None
+    if args.showers_only:
+        args.no_dark, args.dark_forward = True, False
     if args.dark_forward:
         del sh, ctx
         torch.cuda.empty_cache()
@@ -223,20 +386,24 @@ def main():
         "particles_per_step": totals["particles"] / K,
         "particle_steps_per_step": totals["steps"] / K,
         "trials_per_draw": totals["trials"] / max(totals["draws"], 1),
-        "draws_by_process": {name: {"draws": d, "trials_per_draw": t / d}
-                             for name, d, t in zip(("Brem", "Ann", "PairProd", "Comp", "Moller", "Bhabha", "MuonE",
-                                                    "MuonBrem"), last_stats["n_draws_proc"],
-                                                   last_stats["n_trials_proc"])
+        "draws_by_process": {name: {"draws": d, "trials_per_draw": t / d, "maxf_violations": v,
+                                    "clipped_fraction": v / d}
+                             for name, d, t, v in zip(PROC_NAMES, last_stats["n_draws_proc"],
+                                                      last_stats["n_trials_proc"], last_stats["n_viol_proc"])
                              if d > 0},
+        "maxf_violations": last_stats["n_maxf_viol"],
+        "tuning": tuning_report(ctx),
         "kernels": kernels,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     if cpu:
         out["gpu_over_cpu_showers"] = showers_per_s / cpu["showers_per_s"]
-    out["detector_cut"] = det_out
-    out["map_training"] = train_out
-    out["meson_beam"] = meson_out
+    for key, val in (("c1_graphite_10GeV", c1_out), ("c5_400GeV", c5_out), ("detector_cut", det_out),
+                     ("map_training", train_out),
+                     ("meson_beam", meson_out)):
+        if val is not None:
+            out[key] = val
     if dark_out:
         out["dark_forward"] = dark_out
     if yield_out:
@@ -459,6 +626,28 @@ def bench_dark_forward(n=DARK_N, reps=2):
     return out, ds, rec, dark
 
 
+def yield_finite_differences(model, rec, masses, h=1e-5):
+    """dY/dm_V by finite differences of step h m_V at every mass: central
+    between trained masses; at a trained mass (where the interpolation has a
+    kink) second-order one-sided on the side the model differentiates -- the
+    segment that starts there, or for the heaviest trained mass the one that
+    ends there."""
+    masses = np.asarray(masses, dtype=np.float64)
+    knots = np.asarray(model.masses)
+    out = np.empty_like(masses)
+    for k, m in enumerate(masses):
+        at = np.any(np.abs(m / knots - 1) < 1e-12)
+        if not at:
+            Yp, _ = model.evaluate(rec, [m * (1 + h)], grad=False)
+            Ym, _ = model.evaluate(rec, [m * (1 - h)], grad=False)
+            out[k] = (Yp[0] - Ym[0]) / (2 * h * m)
+            continue
+        sgn = -1.0 if m >= knots[-1] * (1 - 1e-12) else 1.0
+        Y = [model.evaluate(rec, [m * (1 + sgn * i * h)], grad=False)[0][0] for i in range(3)]
+        out[k] = sgn * (-3 * Y[0] + 4 * Y[1] - Y[2]) / (2 * h * m)
+    return out
+
+
 def bench_yield(reps=5):
     """Fixed record of 50,000 e- showers (1-20 GeV log-uniform, lead), yield
     and d/dm_V at 32 m_V in [1, 100] MeV; d/d(eps^2) is the yield itself."""
@@ -481,14 +670,9 @@ def bench_yield(reps=5):
     for _ in range(reps):
         Y, dY = model.evaluate(rec, YIELD_MASSES)
     t_g = (time.perf_counter() - t0) / reps
-    # finite differences (step 1e-3 m_V) at masses away from the trained ones
-    inner = [k for k, m in enumerate(YIELD_MASSES)
-             if all(abs(m / t - 1) > 2e-3 for t in model.masses) and model.masses[0] < m < model.masses[-1]]
-    mv = YIELD_MASSES[inner]
-    Yp, _ = model.evaluate(rec, mv * (1 + 1e-3), grad=False)
-    Ym, _ = model.evaluate(rec, mv * (1 - 1e-3), grad=False)
-    fd = (Yp - Ym) / (2e-3 * mv)
-    fd_err = float(np.max(np.abs(fd - dY[inner]) / np.maximum(np.abs(dY[inner]), 1e-300)))
+    fd = yield_finite_differences(model, rec, YIELD_MASSES)
+    fd_err = float(np.max(np.abs(fd - dY) / np.maximum(np.abs(dY), 1e-300)))
+    inner = YIELD_MASSES
     passes = math.ceil(len(YIELD_MASSES) / 16)
     alg = B_YIELD * rec.n * passes
     out = {"workload": f"{YIELD_N} e- showers, E log-uniform [1,20] GeV, {YIELD_MAT}, cutoff {CUTOFF} GeV; "

@@ -115,6 +115,9 @@ typedef struct {
   int64_t launches_transport, launches_draw, launches_emit;
   int64_t n_draws_proc[PETITE_NPROC];  /* draws per process code */
   int64_t n_trials_proc[PETITE_NPROC]; /* candidates per process code */
+  int64_t n_viol_proc[PETITE_NPROC];   /* accepted draws with J*f above max_F per process code (the
+                                        * accept probability min(1, J*f / max_F) clips them, as the
+                                        * reference's test does) */
 } PetiteStats;
 
 /* Create a context: copies the table blobs (host pointers) to the device. */
@@ -144,9 +147,14 @@ typedef struct {
   int32_t sweep_group;       /* sweeps queued per host check (1..64) [16] */
   int32_t group_requests;    /* draws with fewer requests than lanes: requests a lane group is sized to
                               * serve in one launch (1..16) [4] */
-  int32_t pad;
+  int32_t small_sweep;       /* sweeps with fewer particles than this (seen at a host check) run the
+                              * draws of every process in one launch on the caller's stream, with
+                              * launch sizes scaled to the sweep [2^20]; 1: never */
   int64_t sweep_bound;      /* particles per sweep, resumed ones included (>= 64) [2^23] */
   int64_t resume_cap;        /* hand-on records per sweep (1..sweep_bound) [min(sweep_bound, 2^20)] */
+  int32_t small_yield_steps; /* yield_steps of the small sweeps [40; measured: 2^30 (no hand-on)
+                              * is 4 ms per step slower on the 100k lead workload] */
+  int32_t pad;
 } PetiteTuning;
 
 /* Set (tuning != NULL) or reset to the defaults (NULL) the context's launch

@@ -496,19 +496,60 @@ def bound_weights(od, Wa, pid, E0, process):
     return out
 
 
+def interp_with_derivative(x, y, dx, dy, v):
+    """numpy.interp(v, x, y), 0 outside [x0, x_n-1], and its derivative in a
+    parameter that moves the nodes (dx) and values (dy):
+    y'_lo + s (y'_hi - y'_lo) + (y_hi - y_lo) s', s' = -[x'_lo (1-s) + x'_hi s]/(x_hi - x_lo)."""
+    v = np.asarray(v, dtype=np.float64)
+    inside = (v >= x[0]) & (v <= x[-1])
+    lo = np.clip(np.searchsorted(x, v, side="right") - 1, 0, len(x) - 2)
+    xl, xh, yl, yh = x[lo], x[lo + 1], y[lo], y[lo + 1]
+    val = (yh - yl) / (xh - xl) * (v - xl) + yl
+    sw = (v - xl) / (xh - xl)
+    sd = -(dx[lo] * (1.0 - sw) + dx[lo + 1] * sw) / (xh - xl)
+    der = dy[lo] + sw * (dy[lo + 1] - dy[lo]) + (yh - yl) * sd
+    top = v == x[-1]
+    val = np.where(top, y[-1], val)
+    der = np.where(top, dy[-1], der)
+    return np.where(inside, val, 0.0), np.where(inside, der, 0.0)
+
+
 def bound_yield(T, processes, pid, E0, w, mV, table_set="100GeV", rel_step=1e-5, Zeff=29.508, masses="all"):
     """Y and dY/dm_V of the bound-electron processes with tables at each m_V
-    itself; dY/dm_V is the central difference of tables at m_V (1 +- rel_step)."""
+    itself.  The derivative is that of the tables' interpolants at m_V: node
+    positions and values differentiated by central differences of the tables
+    at m_V (1 +- rel_step), the interval weights analytically
+    (interp_with_derivative; DarkComp in log10 space, W = 10^L / den)."""
     procs = [P for P in processes if P in BOUND_PROCESSES]
+    pid = np.asarray(pid)
+    E0 = np.asarray(E0, dtype=np.float64)
     w = np.asarray(w, dtype=np.float64)
     Y, dY = [], []
     for m in np.atleast_1d(mV):
-        vals = []
-        for mm in (m, m * (1 + rel_step), m * (1 - rel_step)):
-            od, Wa = bound_tables(T, mm, table_set, Zeff, masses)
-            vals.append(float(np.sum(sum(bound_weights(od, Wa, pid, E0, P) for P in procs) * w)) if procs else 0.0)
-        Y.append(vals[0])
-        dY.append((vals[1] - vals[2]) / (2 * rel_step * m))
+        (od, Wa), (op, Wp), (om, Wm) = (bound_tables(T, mm, table_set, Zeff, masses)
+                                        for mm in (m, m * (1 + rel_step), m * (1 - rel_step)))
+        f = 1.0 / (2 * rel_step * m)
+        y = dy = 0.0
+        if "DarkAnn" in procs:
+            sel = (pid == -11) & (E0 >= od.min_dark[-11]["DarkAnn"])
+            val, der = interp_with_derivative(Wa[:, 0], Wa[:, 1], (Wp[:, 0] - Wm[:, 0]) * f, (Wp[:, 1] - Wm[:, 1]) * f,
+                                              E0[sel])
+            y += float(np.sum(val * w[sel]))
+            dy += float(np.sum(der * w[sel]))
+        if "DarkComp" in procs:
+            sel = (pid == 22) & (E0 >= od.min_dark[22]["DarkComp"]) & (E0 >= T.min_calc[22])
+            e = E0[sel]
+            den = T.nsigma("PairProd", e) + T.nsigma("Comp", e)
+            ns, nsp, nsm = od.NS["DarkComp"], op.NS["DarkComp"], om.NS["DarkComp"]
+            l = np.log10(e)
+            inside = (l >= ns.lx[0]) & (l <= ns.lx[-1])
+            L, dL = interp_with_derivative(ns.lx, ns.ly, (nsp.lx - nsm.lx) * f, (nsp.ly - nsm.ly) * f, l)
+            L = np.where(inside, L, -20.0)
+            Wc = 10.0 ** L / den
+            y += float(np.sum(Wc * w[sel]))
+            dy += float(np.sum(np.where(inside, Wc * math.log(10.0) * dL, 0.0) * w[sel]))
+        Y.append(y)
+        dY.append(dy)
     return np.array(Y), np.array(dY)
 
 

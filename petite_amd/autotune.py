@@ -37,6 +37,8 @@ SPACE = {
     "min_group_lanes": [2, 4, 8],
     "group_requests": [1, 2, 4],
     "sweep_group": [8, 16, 32],
+    "small_sweep": [1, 4096, 16384, 65536, 262144],
+    "small_yield_steps": [40, 160, 1 << 30],
 }
 FIELDS = {f for f, _ in _lib.Tuning._fields_ if f != "pad"}
 

@@ -555,7 +555,7 @@ __global__ __launch_bounds__(kMesonBlock) void k_meson_decay(DevTables T, Petite
 
 // ------------------------------------------------------ m_V-continuous yield --
 constexpr int kYieldBatch = 16;
-constexpr int kYieldWindow = 4;  // trained masses a pass may use (consecutive)
+constexpr int kYieldWindow = 8;  // trained masses a pass may use (consecutive)
 constexpr double kLn10 = 2.302585092994045684;
 
 struct YieldTables {
@@ -618,30 +618,72 @@ struct BoundView {
   const quad::BoundSlot* slot;
   const double* tab;
   int32_t stride;  // doubles per slot
-  int32_t nv;
-  double inv_2h[kYieldBatch];  // 1 / (2 h m_k)
+  int32_t nv;  // slots per point: 3 with the derivative (m_k, and its node derivatives after k_bound_deriv), else 1
 };
 
-// GetBSMWeights of DarkAnn / DarkComp (dark_shower.py:595-647) with the tables
-// of slot s: the annihilation weight array (linear, 0 outside) and the
-// Compton log-log n sigma over n sigma_PP + n sigma_Comp.
-__device__ inline double bound_w(const DevTables& T, const BoundView& B, int s, int code, const WeightArgs& a,
-                                 double ln_den, double ln_eps2) {
-  const quad::BoundSlot& S = B.slot[s];
-  const double* t = B.tab + (int64_t)s * B.stride;
+// W and dW/dm_V of DarkAnn / DarkComp at point k: the interpolant of slot
+// k * nv at E0, and (GRAD) its derivative from the node derivatives in slot
+// k * nv + 1 (k_bound_deriv).  With s the interval weight,
+//   d/dm [y_lo + s (y_hi - y_lo)] = y'_lo + s (y'_hi - y'_lo) + (y_hi - y_lo) s',
+//   s' = -[x'_lo (1 - s) + x'_hi s] / (x_hi - x_lo);
+// DarkComp in log10 space: W = eps^2 10^L / den, dW = W ln 10 dL.  Below the
+// thresholds or outside the grid W = dW = 0.
+template <bool GRAD>
+__device__ inline void bound_wd(const DevTables& T, const BoundView& B, int k, int code, const WeightArgs& a,
+                                double ln_den, double ln_eps2, double& W, double& dW) {
+  const int s0 = k * B.nv;
+  const quad::BoundSlot& S = B.slot[s0];
+  const double* t = B.tab + (int64_t)s0 * B.stride;
+  const double* d = t + B.stride;
   constexpr int n = quad::kBoundGrid;
+  W = dW = 0.0;
   if (code == DARKANN) {
     const double E0 = a.E0;
-    if (E0 < S.min_ann || !(E0 >= t[0]) || E0 > t[n - 1]) return 0.0;
+    if (E0 < S.min_ann || !(E0 >= t[0]) || E0 > t[n - 1]) return;
     const double* y = t + 3 * n;
-    if (E0 == t[n - 1]) return exp(ln_eps2) * y[n - 1];
+    const double e2 = exp(ln_eps2);
+    if (E0 == t[n - 1]) {
+      W = e2 * y[n - 1];
+      if (GRAD) dW = e2 * d[3 * n + n - 1];
+      return;
+    }
     const int lo = find_interval(t, n, E0, a.lf, S.lnEa0, S.inv_dln_a);
-    const double slope = (y[lo + 1] - y[lo]) / (t[lo + 1] - t[lo]);
-    return exp(ln_eps2) * (slope * (E0 - t[lo]) + y[lo]);
+    const double xl = t[lo], xh = t[lo + 1], yl = y[lo], yh = y[lo + 1];
+    const double slope = (yh - yl) / (xh - xl);
+    W = e2 * (slope * (E0 - xl) + yl);
+    if (GRAD) {
+      const double sw = (E0 - xl) / (xh - xl);
+      const double sd = -(d[lo] * (1.0 - sw) + d[lo + 1] * sw) / (xh - xl);
+      const double dyl = d[3 * n + lo], dyh = d[3 * n + lo + 1];
+      dW = e2 * (dyl + sw * (dyh - dyl) + (yh - yl) * sd);
+    }
+    return;
   }
-  if (a.E0 < S.min_comp || a.E0 < T.d->min_calc[2]) return 0.0;
-  const double L = loglog10(t + 4 * n, t + 5 * n, n, a.l10, S.l0c, S.inv_dlc);
-  return exp(ln_eps2 - ln_den + kLn10 * L);
+  if (a.E0 < S.min_comp || a.E0 < T.d->min_calc[2]) return;
+  const double* lx = t + 4 * n;
+  const double* ly = t + 5 * n;
+  const double l = a.l10;
+  if (l < lx[0] || l > lx[n - 1]) {
+    W = exp(ln_eps2 - ln_den - 20.0 * kLn10);  // the fill 1e-20
+    return;
+  }
+  double L, dL = 0.0;
+  if (l == lx[n - 1]) {
+    L = ly[n - 1];
+    if (GRAD) dL = d[5 * n + n - 1];
+  } else {
+    const int lo = find_interval(lx, n, l, (float)l, S.l0c, S.inv_dlc);
+    const double xl = lx[lo], xh = lx[lo + 1], yl = ly[lo], yh = ly[lo + 1];
+    L = (yh - yl) / (xh - xl) * (l - xl) + yl;
+    if (GRAD) {
+      const double sw = (l - xl) / (xh - xl);
+      const double sd = -(d[4 * n + lo] * (1.0 - sw) + d[4 * n + lo + 1] * sw) / (xh - xl);
+      const double dyl = d[5 * n + lo], dyh = d[5 * n + lo + 1];
+      dL = dyl + sw * (dyh - dyl) + (yh - yl) * sd;
+    }
+  }
+  W = exp(ln_eps2 - ln_den + kLn10 * L);
+  if (GRAD) dW = W * kLn10 * dL;
 }
 
 // One float64 pass over the record: acc[2k] += w W(m_k), acc[2k+1] += w dW/dm(m_k)
@@ -650,7 +692,7 @@ __device__ inline double bound_w(const DevTables& T, const BoundView& B, int s, 
 // w_j)), where both are positive, linearly otherwise.
 template <bool GRAD>
 __global__ __launch_bounds__(256) void k_dark_yield(DevTables T, const YieldTables* __restrict__ Yp, DevStore S,
-                                                    int64_t n, YieldPoints P, BoundView Bv, double* acc) {
+                                                    int64_t n, YieldPoints P, bool bound_on, double* acc) {
   const YieldTables& Y = *Yp;
   double y[kYieldBatch], dy[kYieldBatch];
 #pragma unroll
@@ -666,18 +708,7 @@ __global__ __launch_bounds__(256) void k_dark_yield(DevTables T, const YieldTabl
     for (int a = 0; a < d0.n_active; ++a) {
       const int code = d0.active[a];
       if (code == DARKCOMP ? pid != 22 : family_of(code, pid) < 0) continue;
-      if (Bv.slot && (code == DARKANN || code == DARKCOMP)) {
-#pragma unroll
-        for (int k = 0; k < kYieldBatch; ++k) {
-          if (k >= P.n) break;
-          const int s0 = k * Bv.nv;
-          y[k] += w * bound_w(T, Bv, s0, code, wa, ln_den, P.ln_eps2[0]);
-          if (GRAD)
-            dy[k] += w * (bound_w(T, Bv, s0 + 1, code, wa, ln_den, P.ln_eps2[0]) -
-                          bound_w(T, Bv, s0 + 2, code, wa, ln_den, P.ln_eps2[0])) * Bv.inv_2h[k];
-        }
-        continue;
-      }
+      if (bound_on && (code == DARKANN || code == DARKCOMP)) continue;  // k_bound_yield
       double wv[kYieldWindow], lam[kYieldWindow];
 #pragma unroll
       for (int q = 0; q < kYieldWindow; ++q) {
@@ -689,11 +720,19 @@ __global__ __launch_bounds__(256) void k_dark_yield(DevTables T, const YieldTabl
       for (int k = 0; k < kYieldBatch; ++k) {
         if (k >= P.n) break;
         double lo = 0.0, hi = 0.0, llo = 0.0, lhi = 0.0;
-        switch (P.j[k]) {  // uniform over the grid
-          case 0: lo = wv[0]; llo = lam[0]; hi = wv[1]; lhi = lam[1]; break;
-          case 1: lo = wv[1]; llo = lam[1]; hi = wv[2]; lhi = lam[2]; break;
-          case 2: lo = wv[2]; llo = lam[2]; hi = wv[3]; lhi = lam[3]; break;
-          default: lo = wv[3]; llo = lam[3]; hi = lo; lhi = llo; break;
+        // segment P.j[k] of the window (uniform over the grid); selects
+        // without indexing the register arrays
+#pragma unroll
+        for (int q = 0; q + 1 < kYieldWindow; ++q)
+          if (P.j[k] == q) {
+            lo = wv[q];
+            llo = lam[q];
+            hi = wv[q + 1];
+            lhi = lam[q + 1];
+          }
+        if (P.j[k] >= kYieldWindow - 1) {
+          lo = hi = wv[kYieldWindow - 1];
+          llo = lhi = lam[kYieldWindow - 1];
         }
         if (P.j0 + P.j[k] + 1 >= Y.n_mass) {
           hi = lo;
@@ -726,6 +765,50 @@ __global__ __launch_bounds__(256) void k_dark_yield(DevTables T, const YieldTabl
       if (GRAD) b += shfl_f64(b, (threadIdx.x & 63) ^ off);
     }
     if ((threadIdx.x & 63) == 0 && k < P.n) {
+      unsafeAtomicAdd(acc + 2 * k, a);
+      if (GRAD) unsafeAtomicAdd(acc + 2 * k + 1, b);
+    }
+  }
+}
+
+// The bound-electron processes' part of the yield (DarkAnn of positrons,
+// DarkComp of photons) from the tables at the pass's masses: one float64 pass
+// over the record beside k_dark_yield, which leaves them out, into the same
+// accumulators.  A kernel of its own keeps the interpolation kernel's
+// registers (its three table lookups per mass would double them).
+template <bool GRAD>
+__global__ __launch_bounds__(256) void k_bound_yield(DevTables T, DevStore S, int64_t n, int32_t n_pts, int32_t ann,
+                                                     int32_t comp, double ln_eps2, BoundView Bv, double* acc) {
+  double y[kYieldBatch], dy[kYieldBatch];
+#pragma unroll
+  for (int k = 0; k < kYieldBatch; ++k) y[k] = dy[k] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int pid = S.pid[i];
+    const bool is_ann = ann && pid == -11, is_comp = comp && pid == 22;
+    if (!is_ann && !is_comp) continue;
+    const double E0 = (double)S.p0[i].x;
+    const double w = (double)S.r0[i].w;
+    const WeightArgs wa = weight_args(T, pid, E0);
+    const double ln_den = is_comp ? log(wa.den) : 0.0;
+    const int code = is_ann ? DARKANN : DARKCOMP;
+#pragma unroll
+    for (int k = 0; k < kYieldBatch; ++k) {
+      if (k >= n_pts) break;
+      double W, dW;
+      bound_wd<GRAD>(T, Bv, k, code, wa, ln_den, ln_eps2, W, dW);
+      y[k] += w * W;
+      if (GRAD) dy[k] += w * dW;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kYieldBatch; ++k) {
+    double a = y[k], b = dy[k];
+    for (int off = 32; off > 0; off >>= 1) {
+      a += shfl_f64(a, (threadIdx.x & 63) ^ off);
+      if (GRAD) b += shfl_f64(b, (threadIdx.x & 63) ^ off);
+    }
+    if ((threadIdx.x & 63) == 0 && k < n_pts) {
       unsafeAtomicAdd(acc + 2 * k, a);
       if (GRAD) unsafeAtomicAdd(acc + 2 * k + 1, b);
     }

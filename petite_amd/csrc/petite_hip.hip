@@ -18,6 +18,7 @@
 
 #include "petite_hip.h"
 #include "physics.h"
+#include "sweep.h"
 #include "transport.h"
 
 using namespace petite;
@@ -57,7 +58,8 @@ enum CounterIdx {
   C_STATS = 32,               // [ST_N] run statistics
   C_PTRIALS = 64,             // [PETITE_NPROC] candidates per process
   C_PDRAWS = 80,              // [PETITE_NPROC] draws per process
-  C_N = 96
+  C_PVIOL = 96,               // [PETITE_NPROC] accepted draws above max_F per process
+  C_N = 112
 };
 
 // One launch of the accept/reject kernel.  Requests sit in per-process
@@ -327,6 +329,7 @@ __device__ void draw_loop(const DevTables& T, const DrawLaunch& io, uint64_t see
   wave_stat(io.counters, C_STATS + ST_NOSAMPLE, n_fail);
   wave_stat(io.counters, C_PTRIALS + p, n_trials);
   wave_stat(io.counters, C_PDRAWS + p, n_draw);
+  wave_stat(io.counters, C_PVIOL + p, n_viol);
 }
 
 // One kernel per process: the integrand, the map dimension and the
@@ -352,6 +355,114 @@ typedef void (*DrawKernel)(DevTables, DrawLaunch, uint64_t);
 static const DrawKernel kDrawKernels[PETITE_NPROC] = {
     k_draw<0>, k_draw<1>, k_draw<2>, k_draw<3>, k_draw<4>, k_draw<5>,
     k_draw<6>, k_draw<7>, k_draw<8>, k_draw<9>, k_draw<10>, k_draw<11>};
+
+// One request per block: the 256 threads test candidates j .. j + 255 and
+// the block takes the lowest accepted index (an LDS minimum), the candidate
+// draw_sample's sequential loop accepts.  In the last sweeps of a run, where
+// a process has fewer requests than blocks, a draw's latency is its slowest
+// request's candidate count over the lanes that share it: 256 instead of at
+// most a wave's 64.
+template <int p>
+__device__ void draw_block_one(const DevTables& T, const DrawLaunch& io, uint64_t seed, int64_t r) {
+  constexpr int ND = proc_ndim<p>();
+  __shared__ uint32_t s_win;
+  const PetiteGridDesc& gd = T.d->grid[p];
+  const int32_t ninc[4] = {gd.ninc[0], gd.ninc[1], gd.ninc[2], gd.ninc[3]};
+  const uint32_t max_trials = (uint32_t)(T.d->max_trials < (int64_t)0x7FFFFFF0 ? (T.d->max_trials > 0 ? T.d->max_trials : 1)
+                                                                               : (int64_t)0x7FFFFFF0);
+  const int64_t ridx = (int64_t)p * io.req_cap + r;
+  const int32_t src = io.req_slot[ridx];
+  const int64_t oidx = io.explicit_out ? (int64_t)src : ridx;
+  const double E = io.req_E[ridx];
+  const uint64_t sid = io.req_sid ? io.req_sid[ridx] : io.sid_src[src];
+  const int ie = lu_key(T, p, E);
+  const float* base = T.f32 + gd.node_off + (int64_t)ie * gd.stride;
+  const double mf = T.maxf[p * T.maxE + ie] * T.d->maxF_fudge;
+  for (uint32_t j = draw_possible<p>(T) ? 0u : max_trials; j < max_trials; j += blockDim.x) {
+    if (threadIdx.x == 0) s_win = 0xFFFFFFFFu;
+    __syncthreads();
+    const uint32_t jc = j + threadIdx.x;
+    double x[4] = {0.0, 0.0, 0.0, 0.0};
+    double ua = 1.0, J = 0.0;
+    double w = candidate_w<p, ND>(T, E, base, ninc, seed, sid, jc, x, ua, J);
+    if (ND == 4 && fabs(mf * ua - w) <= kRefBand * w) w = J * integrand_ref<p>(T, E, x);
+    const bool acc = (jc < max_trials) && (mf * ua < w);
+    if (acc) atomicMin(&s_win, jc);
+    __syncthreads();
+    const uint32_t win = s_win;
+    if (win != 0xFFFFFFFFu) {
+      if (jc == win) {
+        float* xo = io.x_out + 4 * oidx;
+        xo[0] = (float)x[0];
+        xo[1] = (float)x[1];
+        xo[2] = (float)x[2];
+        xo[3] = (float)x[3];
+        io.trials_out[oidx] = (int32_t)(jc + 1);
+        io.w_out[oidx] = (float)w;
+        atomicAdd(io.counters + C_STATS + ST_TRIALS, (unsigned long long)(jc + 1));
+        atomicAdd(io.counters + C_STATS + ST_DRAWS, 1ull);
+        atomicAdd(io.counters + C_PTRIALS + p, (unsigned long long)(jc + 1));
+        atomicAdd(io.counters + C_PDRAWS + p, 1ull);
+        if (w > mf) {
+          atomicAdd(io.counters + C_STATS + ST_VIOL, 1ull);
+          atomicAdd(io.counters + C_PVIOL + p, 1ull);
+        }
+      }
+      return;
+    }
+    __syncthreads();  // every thread has read s_win before the next round resets it
+  }
+  if (threadIdx.x == 0) {
+    io.trials_out[oidx] = -(int32_t)max_trials;
+    io.w_out[oidx] = 0.0f;
+    atomicAdd(io.counters + C_STATS + ST_TRIALS, (unsigned long long)max_trials);
+    atomicAdd(io.counters + C_STATS + ST_DRAWS, 1ull);
+    atomicAdd(io.counters + C_STATS + ST_NOSAMPLE, 1ull);
+    atomicAdd(io.counters + C_PTRIALS + p, (unsigned long long)max_trials);
+    atomicAdd(io.counters + C_PDRAWS + p, 1ull);
+  }
+}
+
+// The draws of a small sweep: every SM process in one launch on the
+// caller's stream, blocks [p * per, (p + 1) * per) for process p (bit p of
+// `procs` set); a process with at most `per` requests takes one block per
+// request (draw_block_one).  In the last sweeps of a run, where a handful of requests
+// remain, the per-process launches on their own streams cost more in
+// launches, forks and joins than the draws themselves.  Same draw loop, so
+// the same results.
+__global__ __launch_bounds__(256) void k_draw_small(DevTables T, DrawLaunch io, uint64_t seed, int per, uint32_t procs) {
+  const int p = (int)blockIdx.x / per;
+  const int b = (int)blockIdx.x - p * per;
+  if (p >= 8 || !((procs >> p) & 1u)) return;
+  const int64_t cnt = min((int64_t)io.dcount[p], io.req_cap);
+  if (cnt <= (int64_t)per) {
+    if (b >= cnt) return;
+    switch (p) {
+      case 0: draw_block_one<0>(T, io, seed, b); break;
+      case 1: draw_block_one<1>(T, io, seed, b); break;
+      case 2: draw_block_one<2>(T, io, seed, b); break;
+      case 3: draw_block_one<3>(T, io, seed, b); break;
+      case 4: draw_block_one<4>(T, io, seed, b); break;
+      case 5: draw_block_one<5>(T, io, seed, b); break;
+      case 6: draw_block_one<6>(T, io, seed, b); break;
+      default: draw_block_one<7>(T, io, seed, b); break;
+    }
+    return;
+  }
+  int lg, blocks;
+  draw_plan((double)cnt, per, io.min_lg, io.group_requests, lg, blocks);
+  if (b >= blocks) return;
+  switch (p) {
+    case 0: draw_loop<0>(T, io, seed, cnt, lg); break;
+    case 1: draw_loop<1>(T, io, seed, cnt, lg); break;
+    case 2: draw_loop<2>(T, io, seed, cnt, lg); break;
+    case 3: draw_loop<3>(T, io, seed, cnt, lg); break;
+    case 4: draw_loop<4>(T, io, seed, cnt, lg); break;
+    case 5: draw_loop<5>(T, io, seed, cnt, lg); break;
+    case 6: draw_loop<6>(T, io, seed, cnt, lg); break;
+    default: draw_loop<7>(T, io, seed, cnt, lg); break;
+  }
+}
 
 // Explicit requests into per-process regions (petite_vegas_draw).
 __global__ void k_bucket(DevTables T, const int32_t* proc, const double* E, int64_t n, int32_t* req_slot, double* req_E,
@@ -983,17 +1094,39 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
 // shower.py:471-480): the process's final state in the parent frame from the
 // accepted point xs and the azimuth uniform az, rotated onto the parent's
 // direction (px, py, pz; any length), and their species (process_PIDS, 87-96).
-__device__ inline void vertex_children(int p, int pid, double E, const float* xs, double az, double px, double py,
-                                       double pz, V4* lab, int* cpid) {
+// The accepted point of a 1-D draw in float64.  The draw stores it as float32,
+// but the 1-D kinematics (a cosine near 1 whose complement sets a child's
+// energy, mue_to_mue_fourvecs, kinematics.py:239-265) cancel: a float32 cosine
+// costs ~E * 6e-8 GeV of a knock-on's energy.  The same map transform of the
+// same Philox block (candidate trials - 1 of the parent's stream) gives the
+// float64 value the draw accepted.
+__device__ inline double accepted_x1(const DevTables& T, int p, double E, uint64_t seed, uint64_t sid, int32_t trials,
+                                     float xf) {
+  const PetiteGridDesc& gd = T.d->grid[p];
+  if (gd.ndim != 1 || trials <= 0 || gd.n_E <= 0) return (double)xf;
+  const int ie = lu_key(T, p, E);
+  const float* base = T.f32 + gd.node_off + (int64_t)ie * gd.stride;
+  const U4 a = draw_block(seed, sid, (uint32_t)(trials - 1), PURPOSE_VEGAS_A);
+  const uint32_t rr[4] = {a.x, a.y, a.z, a.w};
+  const int32_t ninc[4] = {gd.ninc[0], gd.ninc[1], gd.ninc[2], gd.ninc[3]};
+  double x[4] = {0.0, 0.0, 0.0, 0.0};
+  map_point_n<1>(base, ninc, rr, x);
+  return x[0];
+}
+
+// x0: the 1-D processes' variable in float64 (accepted_x1); xs: the 4-D
+// processes' point.
+__device__ inline void vertex_children(int p, int pid, double E, const float* xs, double x0, double az, double px,
+                                       double py, double pz, V4* lab, int* cpid) {
   const double m = pid_mass(pid);
   V4 a, b;
   switch (p) {
     case BREM: case MUONBREM: kin_brem(E, m, xs, az, a, b); break;
     case PAIRPROD: kin_pairprod(E, xs, az, a, b); break;
-    case COMP: kin_compton(E, 0.0, (double)xs[0], az, a, b); break;
-    case ANN: kin_ann(E, 0.0, (double)xs[0], az, a, b); break;
-    case MUONE: kin_mue(E, (double)xs[0], az, a, b); break;
-    default: kin_ee(E, (double)xs[0], az, a, b); break;  // Moller, Bhabha
+    case COMP: kin_compton(E, 0.0, x0, az, a, b); break;
+    case ANN: kin_ann(E, 0.0, x0, az, a, b); break;
+    case MUONE: kin_mue(E, x0, az, a, b); break;
+    default: kin_ee(E, x0, az, a, b); break;  // Moller, Bhabha
   }
   child_pids(p, pid, &cpid[0], &cpid[1]);
   const double pn = sqrt(px * px + py * py + pz * pz);
@@ -1068,7 +1201,8 @@ __global__ __launch_bounds__(256) void k_emit(DevTables T, DevStore S, EmitLaunc
       sid = io.par.sid[oidx];
       const double az = u01d(draw_block(seed, sid, 0, PURPOSE_INTERACT).y);
       const float4 pf = io.par.pf[oidx];
-      vertex_children(p, pid, E, xs, az, pf.y, pf.z, pf.w, lab, cpid);
+      const double x0 = accepted_x1(T, p, E, seed, sid, tr, xs[0]);
+      vertex_children(p, pid, E, xs, x0, az, pf.y, pf.z, pf.w, lab, cpid);
       for (int k = 0; k < 2; ++k) {
         keep[k] = lab[k].e > T.d->min_energy;
         if (!keep[k] && shower_energy) unsafeAtomicAdd(shower_energy + 4 * shower + 2, lab[k].e);
@@ -1139,7 +1273,7 @@ __global__ void k_vertex_requests(DevTables T, const PetiteVertex* v, const int3
 
 // Children 2i and 2i+1 of vertex i (child index k = 0, 1), every child kept
 // (sample_scattering returns both; generate_shower's cut is the caller's).
-__global__ void k_vertex_emit(const PetiteVertex* v, const int32_t* proc, int64_t n, const float* x_out,
+__global__ void k_vertex_emit(DevTables T, const PetiteVertex* v, const int32_t* proc, int64_t n, const float* x_out,
                               const int32_t* trials, uint64_t seed, DevStore out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1150,7 +1284,8 @@ __global__ void k_vertex_emit(const PetiteVertex* v, const int32_t* proc, int64_
   int cpid[2] = {0, 0};
   if (ok) {
     const double az = u01d(draw_block(seed, q.sid, 0, PURPOSE_INTERACT).y);
-    vertex_children(p, q.pid, q.E, x_out + 4 * i, az, q.px, q.py, q.pz, lab, cpid);
+    const double x0 = accepted_x1(T, p, q.E, seed, q.sid, trials[i], x_out[4 * i]);
+    vertex_children(p, q.pid, q.E, x_out + 4 * i, x0, az, q.px, q.py, q.pz, lab, cpid);
   }
   for (int k = 0; k < 2; ++k) {
     const int64_t c = 2 * i + k;
@@ -1177,25 +1312,15 @@ __global__ void k_sweep_setup(unsigned long long* ctr, int64_t* range, unsigned 
   if (t < PETITE_NPROC) req_count[t] = 0u;
   if (t <= C_TRANSPORT) ctr[t] = 0ull;  // draw refill and transport refill counters
   if (t == 0) {
-    // the previous transport's hand-ons; a reservation past the buffer kept
-    // its particle stepping, so only the first resume_cap records exist
-    int64_t nres = (int64_t)ctr[C_YIELD_OUT];
-    if (nres > resume_cap) nres = resume_cap;
-    // new particles fill the sweep up to `bound` with the resumed ones, so
-    // that no process gets more draw requests than its region holds
-    const int64_t first = range[2];
-    int64_t now = (int64_t)ctr[C_STORE];
-    if (now > capacity) now = capacity;
-    int64_t n = now - first;
-    if (n > bound - nres) n = bound - nres;
-    if (n < 0) n = 0;
-    range[0] = first;
-    range[1] = n;
-    range[2] = first + n;
+    const SweepRange sr = sweep_range(range[2], (int64_t)ctr[C_STORE], capacity, bound, (int64_t)ctr[C_YIELD_OUT],
+                                      resume_cap);
+    range[0] = sr.first;
+    range[1] = sr.n;
+    range[2] = sr.first + sr.n;
     ctr[C_YIELD_OUT] = 0ull;
     ctr[C_RPARITY] ^= 1ull;
-    range[3] = nres;
-    if (n > 0 || nres > 0) ctr[C_SWEEPS] += 1ull;
+    range[3] = sr.nres;
+    if (sr.n > 0 || sr.nres > 0) ctr[C_SWEEPS] += 1ull;
   }
 }
 
@@ -1330,6 +1455,7 @@ __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 // Clears the counter block and the request counts of the allocation-free
 // draws (a kernel rather than a memset, so the call stays capturable).
+static_assert(C_N <= 128, "k_noalloc_clear clears the counters with one 128-thread block");
 __global__ void k_noalloc_clear(unsigned long long* ctr, unsigned int* counts) {
   const int t = threadIdx.x;
   if (t < C_N) ctr[t] = 0ull;
@@ -1447,6 +1573,8 @@ static PetiteTuning default_tuning() {
   t.min_group_lanes = 4;
   t.sweep_group = 16;
   t.group_requests = 4;
+  t.small_sweep = 1 << 20;
+  t.small_yield_steps = 40;
   t.sweep_bound = (int64_t)1 << 23;
   t.resume_cap = (int64_t)1 << 20;
   return t;
@@ -1765,6 +1893,8 @@ int petite_set_tuning(PetiteCtx* c, const PetiteTuning* in) {
   t.group_requests = pick(t.group_requests, d.group_requests);
   t.sweep_bound = pick(t.sweep_bound, d.sweep_bound);
   t.resume_cap = pick(t.resume_cap, std::min<int64_t>(d.resume_cap, t.sweep_bound));
+  t.small_sweep = pick(t.small_sweep, d.small_sweep);
+  t.small_yield_steps = pick(t.small_yield_steps, d.small_yield_steps);
   t.pad = 0;
   if (t.batch_lanes > 64) return fail(PETITE_ERR_ARG, "petite_set_tuning: batch_lanes above 64");
   if (t.transport_waves > 64 || t.emit_waves > 64) return fail(PETITE_ERR_ARG, "petite_set_tuning: more than 64 waves per CU");
@@ -2009,7 +2139,21 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   const unsigned tr_blocks = blocks_for(c, tn.transport_waves);
   const unsigned em_blocks = blocks_for(c, tn.emit_waves);
   const TransportParams TP{tn.batch_lanes, (uint32_t)tn.yield_steps};
+  const TransportParams TP_small{tn.batch_lanes, (uint32_t)tn.small_yield_steps};
   const int group = tn.sweep_group;
+  // small sweeps (the deep generations at the end of a run): one block per CU
+  // for the transport and the emission, and the draws of every process in
+  // one launch (k_draw_small); set from the last sweep of each host check
+  bool small = false;
+  // small-sweep launch sizes, from the size of the last sweep the host saw:
+  // about one transport / emission lane per particle (at least a block per
+  // CU), and draw blocks per process for about 32 requests per block (at
+  // least 64, at most a big sweep's Brem blocks)
+  unsigned small_tr = (unsigned)c->n_cu, small_em = (unsigned)c->n_cu;
+  int small_per = 64;
+  uint32_t small_procs = 0;
+  for (int p = 0; p < 8; ++p)
+    if (c->desc.grid[p].n_E > 0 && (muons || (p != MUONE && p != MUONBREM))) small_procs |= 1u << p;
   const size_t lds = (size_t)c->dev.lds_doubles * sizeof(double);
   if (c->timing) {
     while ((int)c->tev.size() < 4 * group) {
@@ -2028,16 +2172,23 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
                          rcap);
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 0], s));
-      hipLaunchKernelGGL(k_transport, dim3(tr_blocks), dim3(256), lds, s, c->dev, S, (int64_t)0, (int64_t)0, seed, R,
+      hipLaunchKernelGGL(k_transport, dim3(small ? small_tr : tr_blocks), dim3(256), lds, s, c->dev, S,
+                         (int64_t)0, (int64_t)0, seed, R,
                          c->par, shower_energy, ctr, c->dsig_lds, (const int64_t*)c->drange,
-                         ResumeBuf{c->dresume, rcap}, TP);
+                         ResumeBuf{c->dresume, rcap}, small ? TP_small : TP);
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 1], s));
-      rc = launch_draws_device(c, io, seed, s, muons);
-      if (rc) return rc;
+      if (small) {
+        hipLaunchKernelGGL(k_draw_small, dim3((unsigned)(8 * small_per)), dim3(256), 0, s, c->dev, io, seed, small_per,
+                           small_procs);
+        HIP_TRY(hipGetLastError());
+      } else {
+        rc = launch_draws_device(c, io, seed, s, muons);
+        if (rc) return rc;
+      }
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 2], s));
-      hipLaunchKernelGGL(k_emit, dim3(em_blocks), dim3(256), 0, s, c->dev, S, em, seed, ctr + C_STORE, shower_energy,
-                         ctr + C_STATS);
+      hipLaunchKernelGGL(k_emit, dim3(small ? small_em : em_blocks), dim3(256), 0, s, c->dev, S, em, seed,
+                         ctr + C_STORE, shower_energy, ctr + C_STATS);
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 3], s));
     }
@@ -2055,6 +2206,14 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
         HIP_TRY(hipEventElapsedTime(&ms, c->tev[4 * g + 2], c->tev[4 * g + 3]));
         t_em += ms;
       }
+    }
+    {
+      const int64_t last = h_range[1] + h_range[3];
+      small = last < (int64_t)tn.small_sweep;
+      const int64_t lanes_blocks = (last + 255) / 256;
+      small_tr = (unsigned)std::min<int64_t>(tr_blocks, std::max<int64_t>(c->n_cu, lanes_blocks));
+      small_em = (unsigned)std::min<int64_t>(em_blocks, std::max<int64_t>(c->n_cu, lanes_blocks));
+      small_per = (int)std::min<int64_t>(std::max(64, draw_blocks(c, BREM)), std::max<int64_t>(64, (last + 31) / 32));
     }
     if ((int64_t)h_store > store->capacity) break;  // children were dropped: report the size reached
     if (h_range[1] == 0 && h_range[3] == 0 && (int64_t)h_store == h_range[2]) break;  // the last sweep found nothing to do
@@ -2074,11 +2233,13 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   stats->n_iterations = (int64_t)sweeps;
   stats->n_dropped = (int64_t)hs[ST_DROPPED];
   {
-    unsigned long long pt[PETITE_NPROC], pd[PETITE_NPROC];
+    unsigned long long pt[PETITE_NPROC], pd[PETITE_NPROC], pv[PETITE_NPROC];
     HIP_TRY(hipMemcpyAsync(pt, ctr + C_PTRIALS, sizeof(pt), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(pd, ctr + C_PDRAWS, sizeof(pd), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(pv, ctr + C_PVIOL, sizeof(pv), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     for (int p = 0; p < PETITE_NPROC; ++p) {
+      stats->n_viol_proc[p] = (int64_t)pv[p];
       stats->n_trials_proc[p] = (int64_t)pt[p];
       stats->n_draws_proc[p] = (int64_t)pd[p];
     }
@@ -2136,7 +2297,7 @@ int petite_sample_scattering(PetiteCtx* c, const PetiteVertex* v, const int32_t*
   if (rc) return rc;
   DevStore O{(float4*)out->p0, (float4*)out->r0, (float4*)out->pf, (float4*)out->rf, out->pid, out->parent, out->meta,
              out->shower, out->sid, out->capacity};
-  hipLaunchKernelGGL(k_vertex_emit, dim3(nb), dim3(256), 0, s, v, proc, n, (const float*)c->res_x,
+  hipLaunchKernelGGL(k_vertex_emit, dim3(nb), dim3(256), 0, s, c->dev, v, proc, n, (const float*)c->res_x,
                      (const int32_t*)trials_out, seed, O);
   HIP_TRY(hipGetLastError());
   unsigned long long st[ST_N];
@@ -2205,6 +2366,24 @@ static int check_dark_desc(const PetiteDarkDesc* desc, const double* blob, int64
   return PETITE_OK;
 }
 
+// (g0, inv) of an interval guess (g - g0) * inv for an ascending grid in g =
+// f(x) (ln or log10): used when every node lies within one interval of its
+// guessed place, else inv = 0 (bisection).
+static void grid_guess(const double* x, int n, bool log10_values, double& g0, double& inv) {
+  g0 = 0.0;
+  inv = 0.0;
+  if (n < 3) return;
+  auto g = [&](double v) { return log10_values ? v : std::log(v); };
+  if (!log10_values && !(x[0] > 0.0)) return;
+  const double a = g(x[0]), b = g(x[n - 1]);
+  if (!(b > a)) return;
+  const double iv = (double)(n - 1) / (b - a);
+  for (int i = 0; i < n; ++i)
+    if (std::fabs((g(x[i]) - a) * iv - (double)i) > 0.5) return;
+  g0 = a;
+  inv = iv;
+}
+
 int petite_dark_quadrature(const PetiteDarkQuadJob* jobs, int32_t n_jobs, const double* blob, int64_t n_blob,
                            double* out, int64_t n_out) {
   if (n_jobs < 0 || (n_jobs > 0 && (!jobs || !blob || !out)) || n_blob < 0 || n_out < 0)
@@ -2262,6 +2441,7 @@ int petite_dark_quadrature(const PetiteDarkQuadJob* jobs, int32_t n_jobs, const 
     J.lx = d_blob + q.ns.off;
     J.ly = d_blob + q.ns.off + q.ns.n;
     J.n_ns = q.ns.n;
+    grid_guess(blob + q.ns.off, q.ns.n, true, J.ns_l0, J.ns_inv);
     J.zero_small = q.zero_small ? 1 : 0;
     J.n_loss = q.n_loss;
     J.n_mfp = q.n_mfp;
@@ -2272,6 +2452,8 @@ int petite_dark_quadrature(const PetiteDarkQuadJob* jobs, int32_t n_jobs, const 
       J.loss_n[c] = on ? q.loss[c].n : 0;
       J.grid[c] = on ? d_blob + q.grid[c].off : nullptr;
       J.grid_n[c] = on ? q.grid[c].n : 0;
+      J.grid_ln0[c] = J.grid_inv[c] = 0.0;
+      if (on) grid_guess(blob + q.grid[c].off, q.grid[c].n, false, J.grid_ln0[c], J.grid_inv[c]);
       J.cum[c] = on ? d_blob + cum_at : nullptr;
       if (on) cum_at += q.grid[c].n;
       const bool mon = c < q.n_mfp;
@@ -2550,6 +2732,7 @@ int petite_yield_bound(PetiteCtx* c, const PetiteBoundDesc* d, const double* blo
       J.loss_n[q] = d->loss[q].n;
       J.grid[q] = J.loss_x[q];
       J.grid_n[q] = d->loss[q].n;
+      grid_guess(blob + d->loss[q].off, d->loss[q].n, false, J.grid_ln0[q], J.grid_inv[q]);
       J.cum[q] = cum[q];
       J.mfp_x[q] = c->bnd_blob + d->mfp[q].off;
       J.mfp_y[q] = c->bnd_blob + d->mfp[q].off + d->mfp[q].n;
@@ -2668,7 +2851,6 @@ int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t
           BP.top_ann[sl] = c->bnd_top_ann[j];
           BP.top_comp[sl] = c->bnd_top_comp[j];
         }
-        Bv.inv_2h[k] = 1.0 / (2.0 * c->bnd_h * m);
       }
       hipLaunchKernelGGL(quad::k_bound_tables, dim3((unsigned)BP.n_slots), dim3(256), 0, s, c->bnd, BP, c->bnd_slots,
                          c->bnd_tab, c->bnd_jobs);
@@ -2681,6 +2863,14 @@ int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t
                          dim3(64 * quad::kWavesPerBlock), 0, s, (const quad::Job*)c->bnd_jobs, BP.n_slots, n_int,
                          c->bnd_bad);
       HIP_TRY(hipGetLastError());
+      if (nv == 3) {
+        quad::BoundDeriv D{};
+        D.n_pts = P.n;
+        D.stride = quad::kBoundTab + c->bnd.node_stride;
+        for (int k = 0; k < P.n; ++k) D.inv_2h[k] = 1.0 / (2.0 * c->bnd_h * mV[order[k0 + k]]);
+        hipLaunchKernelGGL(quad::k_bound_deriv, dim3((unsigned)P.n), dim3(256), 0, s, D, c->bnd_tab);
+        HIP_TRY(hipGetLastError());
+      }
       Bv.slot = c->bnd_slots;
       Bv.tab = c->bnd_tab;
       Bv.stride = quad::kBoundTab + c->bnd.node_stride;
@@ -2688,13 +2878,28 @@ int petite_dark_yield(PetiteCtx* c, const PetiteStore* sm, int64_t n_sm, int32_t
     }
     HIP_TRY(hipMemsetAsync(c->yacc, 0, 2 * dark::kYieldBatch * sizeof(double), s));
     if (n_sm > 0) {
+      const bool bound_on = Bv.slot != nullptr;
       if (dyield_host)
         hipLaunchKernelGGL(dark::k_dark_yield<true>, dim3((unsigned)blocks), dim3(256), 0, s, c->dev,
-                           (const dark::YieldTables*)c->ytab_dev, S, n_sm, P, Bv, c->yacc);
+                           (const dark::YieldTables*)c->ytab_dev, S, n_sm, P, bound_on, c->yacc);
       else
         hipLaunchKernelGGL(dark::k_dark_yield<false>, dim3((unsigned)blocks), dim3(256), 0, s, c->dev,
-                           (const dark::YieldTables*)c->ytab_dev, S, n_sm, P, Bv, c->yacc);
+                           (const dark::YieldTables*)c->ytab_dev, S, n_sm, P, bound_on, c->yacc);
       HIP_TRY(hipGetLastError());
+      if (bound_on) {
+        int ann = 0, comp = 0;
+        for (int a = 0; a < c->ytab.d[0].n_active; ++a) {
+          ann |= c->ytab.d[0].active[a] == DARKANN;
+          comp |= c->ytab.d[0].active[a] == DARKCOMP;
+        }
+        if (dyield_host)
+          hipLaunchKernelGGL(dark::k_bound_yield<true>, dim3((unsigned)blocks), dim3(256), 0, s, c->dev, S, n_sm, P.n,
+                             ann, comp, P.ln_eps2[0], Bv, c->yacc);
+        else
+          hipLaunchKernelGGL(dark::k_bound_yield<false>, dim3((unsigned)blocks), dim3(256), 0, s, c->dev, S, n_sm, P.n,
+                             ann, comp, P.ln_eps2[0], Bv, c->yacc);
+        HIP_TRY(hipGetLastError());
+      }
     }
     double h[2 * dark::kYieldBatch];
     HIP_TRY(hipMemcpyAsync(h, c->yacc, sizeof(h), hipMemcpyDeviceToHost, s));

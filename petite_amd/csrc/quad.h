@@ -42,6 +42,8 @@ __constant__ double kGLW[8] = {0.10122853629037669, 0.22238103445337434, 0.31370
 struct Job {
   const double* lx;          // log10 E of the log-log n sigma table
   const double* ly;          // log10 n sigma (cm^-1)
+  double ns_l0, ns_inv;      // lx[0] and (n-1)/(lx[n-1]-lx[0]) when lx is near uniform (inv = 0: bisect)
+  double grid_ln0[3], grid_inv[3];  // the same for ln of each survival grid
   int32_t n_ns;
   int32_t zero_small;        // n sigma < 1e-18 counts as 0 (all but bound-electron DarkAnn)
   int32_t n_loss;            // survival channels (<= 3)
@@ -80,16 +82,42 @@ __host__ __device__ inline double lin0(const double* x, const double* y, int n, 
   return slope * (v - x[lo]) + y[lo];
 }
 
+// Interval lo with x[lo] <= v < x[lo+1] of an ascending grid, v inside
+// [x0, x_{n-1}): from the guess (g - g0) * inv on a near-uniform grid (in
+// g = ln v or log10 v), corrected against the nodes; bisection when inv = 0.
+__device__ inline int interval(const double* x, int n, double v, double g, double g0, double inv) {
+  int lo;
+  if (inv > 0.0) {
+    const double t = (g - g0) * inv;
+    lo = t < 0.0 ? 0 : (t > (double)(n - 2) ? n - 2 : (int)t);
+    while (lo > 0 && x[lo] > v) --lo;
+    while (lo < n - 2 && x[lo + 1] <= v) ++lo;
+    return lo;
+  }
+  lo = 0;
+  int hi = n - 1;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (x[mid] <= v) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// numpy.interp inside the grid, 0 outside, with an interval guess from lnv.
+__device__ inline double lin0g(const double* x, const double* y, int n, double v, double lnv, double ln0, double inv) {
+  if (n <= 0 || v < x[0] || v > x[n - 1]) return 0.0;
+  if (v == x[n - 1]) return y[n - 1];
+  const int lo = interval(x, n, v, lnv, ln0, inv);
+  const double slope = (y[lo + 1] - y[lo]) / (x[lo + 1] - x[lo]);
+  return slope * (v - x[lo]) + y[lo];
+}
+
 // interpolate1d in log10-log10 with fill 1e-20 (dark_shower.py:31-46).
-__device__ inline double loglog_val(const double* lx, const double* ly, int n, double E) {
+__device__ inline double loglog_val(const double* lx, const double* ly, int n, double E, double l0, double inv) {
   const double l = log10(E);
   if (n <= 0 || l < lx[0] || l > lx[n - 1]) return 1.0e-20;
   if (l == lx[n - 1]) return pow(10.0, ly[n - 1]);
-  int lo = 0, hi = n - 1;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (lx[mid] <= l) lo = mid; else hi = mid;
-  }
+  const int lo = interval(lx, n, l, l, l0, inv);
   const double slope = (ly[lo + 1] - ly[lo]) / (lx[lo + 1] - lx[lo]);
   return pow(10.0, slope * (l - lx[lo]) + ly[lo]);
 }
@@ -174,15 +202,17 @@ __global__ void k_cumint(const Job* __restrict__ jobs, int n_jobs) {
 }
 
 __device__ inline double survival_arg(const Job& J, const double* iiEi, double E) {
+  const double lnE = (double)__logf((float)E);  // interval guess only
   double d = 0.0;
-  for (int c = 0; c < J.n_loss; ++c) d += iiEi[c] - lin0(J.grid[c], J.cum[c], J.grid_n[c], E);
+  for (int c = 0; c < J.n_loss; ++c)
+    d += iiEi[c] - lin0g(J.grid[c], J.cum[c], J.grid_n[c], E, lnE, J.grid_ln0[c], J.grid_inv[c]);
   return d;
 }
 
 // n sigma(E) / (dE/dx) * S(E, Ei)   (dark_shower.py:311-336 integrands)
 __device__ inline double integrand(const Job& J, const double* iiEi, double E, double Ei) {
   const double k = J.dEdx * kCmToM;
-  double n = loglog_val(J.lx, J.ly, J.n_ns, E);
+  double n = loglog_val(J.lx, J.ly, J.n_ns, E, J.ns_l0, J.ns_inv);
   if (J.zero_small && n < 1.0e-18) n = 0.0;
   const double d = survival_arg(J, iiEi, E);
   const double s = (d < 0.0 || E > Ei) ? 0.0 : exp(-d / J.dEdx / kCmToM);
@@ -491,26 +521,55 @@ __global__ __launch_bounds__(256) void k_bound_tables(BoundParams B, BoundPass P
     S.l0c = lc0;
     S.inv_dlc = (double)(kBoundGrid - 1) / (lc1 - lc0);
     jobs[s].lo = Ea0;
+    // lx = log10(Ea + 1e-20) of a log-spaced grid: near uniform
+    jobs[s].ns_l0 = log10(Ea0 + 1e-20);
+    jobs[s].ns_inv = (double)(kBoundGrid - 1) / (log10(Ea1 + 1e-20) - log10(Ea0 + 1e-20));
+  }
+}
+
+// Derivative tables of the pass's masses: for point k, the slots m_k (1 + h)
+// (3k + 1) and m_k (1 - h) (3k + 2) become, in place in slot 3k + 1, the
+// central differences d/dm_V of the grid energies, the weight array, and the
+// Compton log-log table (nodes and values) at m_k.  With them the yield
+// differentiates the interpolant of slot 3k itself: the node values and
+// positions move with m_V, the interval weights follow analytically.
+struct BoundDeriv {
+  int32_t n_pts;
+  int32_t stride;
+  double inv_2h[16];  // 1 / (2 h m_k)
+};
+
+__global__ void k_bound_deriv(BoundDeriv D, double* __restrict__ tab) {
+  const int k = blockIdx.x;
+  const int i = threadIdx.x;
+  if (k >= D.n_pts || i >= kBoundGrid) return;
+  double* p = tab + (int64_t)(3 * k + 1) * D.stride;
+  const double* m = tab + (int64_t)(3 * k + 2) * D.stride;
+  const double f = D.inv_2h[k];
+  for (const int a : {0, 3, 4, 5}) {  // Ea, W, lxc, lyc
+    const int j = a * kBoundGrid + i;
+    p[j] = (p[j] - m[j]) * f;
   }
 }
 
 // Panel nodes of each slot: the survival grids (`fixed`, ascending) merged
-// with the nodes of the slot's log-log table (10 ** lx), duplicates once.
+// with the nodes of the slot's log-log table (its grid energies Ea; 10 ** lx
+// differs from them by rounding only), duplicates once.
 __global__ void k_bound_nodes(BoundParams B, int n_slots, const double* __restrict__ fixed, double* __restrict__ tab,
                               Job* __restrict__ jobs) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_slots) return;
   double* t = tab + (int64_t)s * (kBoundTab + B.node_stride);
-  const double* lx = t + kBoundGrid;
+  const double* Ea = t;
   double* out = t + kBoundTab;
   int i = 0, j = 0, n = 0;
   double last = -1.0;
   while (i < B.n_static || j < kBoundGrid) {
     double v;
-    if (j >= kBoundGrid || (i < B.n_static && fixed[i] <= pow(10.0, lx[j]))) {
+    if (j >= kBoundGrid || (i < B.n_static && fixed[i] <= Ea[j])) {
       v = fixed[i++];
     } else {
-      v = pow(10.0, lx[j++]);
+      v = Ea[j++];
     }
     if (n == 0 || v != last) out[n++] = v;
     last = v;

@@ -97,3 +97,20 @@ def test_bound_yield_derivative_is_a_consistent_central_difference(setup):
     Yp, _ = bound_yield(T, ["DarkAnn", "DarkComp"], pid, E0, w, [m * (1 + h)])
     Ym, _ = bound_yield(T, ["DarkAnn", "DarkComp"], pid, E0, w, [m * (1 - h)])
     assert abs(dY[0] / ((Yp[0] - Ym[0]) / (2 * h * m)) - 1) < 1e-5
+
+
+def test_interpolant_derivative_exact_for_linear_tables():
+    # nodes x(m) = x0 (1 + m), values y = a(m) x + b(m): the interpolant is
+    # the line itself at any m, so its derivative at fixed v is a'(m) v + b'(m)
+    from oracle.dark import interp_with_derivative
+    m = 0.3
+    x0 = np.array([1.0, 1.7, 2.9, 4.0, 6.5])
+    a, da, b, db = 2.0 + m * m, 2 * m, 0.5 - m, -1.0
+    x, dx = x0 * (1 + m), x0
+    y, dy = a * x + b, da * x + a * dx + db
+    v = np.array([1.3, 2.0, 3.3, 7.4, 8.45, 0.5, 9.0])
+    val, der = interp_with_derivative(x, y, dx, dy, v)
+    ins = (v >= x[0]) & (v <= x[-1])
+    np.testing.assert_allclose(val[ins], a * v[ins] + b, rtol=1e-13)
+    np.testing.assert_allclose(der[ins], da * v[ins] + db, rtol=1e-12, atol=1e-12)
+    assert np.all(val[~ins] == 0) and np.all(der[~ins] == 0)
