@@ -154,7 +154,8 @@ typedef struct {
   int64_t resume_cap;        /* hand-on records per sweep (1..sweep_bound) [min(sweep_bound, 2^20)] */
   int32_t small_yield_steps; /* yield_steps of the small sweeps [40; measured: 2^30 (no hand-on)
                               * is 4 ms per step slower on the 100k lead workload] */
-  int32_t pad;
+  int32_t graph;             /* 1: a sweep group with per-process draw streams is captured once per
+                              * run as a HIP graph and replayed; 2: direct launches [2] */
 } PetiteTuning;
 
 /* Set (tuning != NULL) or reset to the defaults (NULL) the context's launch

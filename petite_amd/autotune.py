@@ -39,6 +39,7 @@ SPACE = {
     "sweep_group": [8, 16, 32],
     "small_sweep": [1, 4096, 16384, 65536, 262144],
     "small_yield_steps": [40, 160, 1 << 30],
+    "graph": [1, 2],
 }
 FIELDS = {f for f, _ in _lib.Tuning._fields_ if f != "pad"}
 

@@ -1528,6 +1528,7 @@ struct PetiteCtx {
   quad::Job* bnd_jobs = nullptr;
   int* bnd_bad = nullptr;
   const double* bnd_fixed = nullptr;   // survival-grid panel nodes (in bnd_blob)
+  hipStream_t cstream = nullptr;       // capture stream of the sweep-group graphs (default priority)
 };
 
 static void free_entries(PetiteCtx* c) {
@@ -1575,6 +1576,7 @@ static PetiteTuning default_tuning() {
   t.group_requests = 4;
   t.small_sweep = 1 << 20;
   t.small_yield_steps = 40;
+  t.graph = 2;
   t.sweep_bound = (int64_t)1 << 23;
   t.resume_cap = (int64_t)1 << 20;
   return t;
@@ -1856,6 +1858,7 @@ void petite_ctx_destroy(PetiteCtx* c) {
   for (hipEvent_t& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->dfork) (void)hipEventDestroy(c->dfork);
+  if (c->cstream) (void)hipStreamDestroy(c->cstream);
   for (hipEvent_t e : c->tev)
     if (e) (void)hipEventDestroy(e);
   if (c->drange) (void)hipFree(c->drange);
@@ -1895,7 +1898,8 @@ int petite_set_tuning(PetiteCtx* c, const PetiteTuning* in) {
   t.resume_cap = pick(t.resume_cap, std::min<int64_t>(d.resume_cap, t.sweep_bound));
   t.small_sweep = pick(t.small_sweep, d.small_sweep);
   t.small_yield_steps = pick(t.small_yield_steps, d.small_yield_steps);
-  t.pad = 0;
+  t.graph = pick(t.graph, d.graph);
+  if (t.graph > 2) return fail(PETITE_ERR_ARG, "petite_set_tuning: graph is 1 (capture) or 2 (direct launches)");
   if (t.batch_lanes > 64) return fail(PETITE_ERR_ARG, "petite_set_tuning: batch_lanes above 64");
   if (t.transport_waves > 64 || t.emit_waves > 64) return fail(PETITE_ERR_ARG, "petite_set_tuning: more than 64 waves per CU");
   for (int p = 0; p < 8; ++p)
@@ -2166,31 +2170,81 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   int64_t queued = 0;
   unsigned long long h_store = n0;
   int64_t h_range[4] = {0, 0, 0, 0};
-  while (true) {
+  // One group of sweeps.  Its launches take the same arguments every time
+  // (sizes live on the device), so with tn.graph a group that uses the
+  // per-process draw streams is captured once per run as a HIP graph, fork
+  // and join events included, and replayed for the later such groups.
+  auto enqueue_group = [&](bool sm, hipStream_t s) -> int {
     for (int g = 0; g < group; ++g) {
       hipLaunchKernelGGL(k_sweep_setup, dim3(1), dim3(64), 0, s, ctr, c->drange, c->req_count, store->capacity, bound,
                          rcap);
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 0], s));
-      hipLaunchKernelGGL(k_transport, dim3(small ? small_tr : tr_blocks), dim3(256), lds, s, c->dev, S,
+      hipLaunchKernelGGL(k_transport, dim3(sm ? small_tr : tr_blocks), dim3(256), lds, s, c->dev, S,
                          (int64_t)0, (int64_t)0, seed, R,
                          c->par, shower_energy, ctr, c->dsig_lds, (const int64_t*)c->drange,
-                         ResumeBuf{c->dresume, rcap}, small ? TP_small : TP);
+                         ResumeBuf{c->dresume, rcap}, sm ? TP_small : TP);
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 1], s));
-      if (small) {
+      if (sm) {
         hipLaunchKernelGGL(k_draw_small, dim3((unsigned)(8 * small_per)), dim3(256), 0, s, c->dev, io, seed, small_per,
                            small_procs);
         HIP_TRY(hipGetLastError());
       } else {
-        rc = launch_draws_device(c, io, seed, s, muons);
-        if (rc) return rc;
+        const int rd = launch_draws_device(c, io, seed, s, muons);
+        if (rd) return rd;
       }
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 2], s));
-      hipLaunchKernelGGL(k_emit, dim3(small ? small_em : em_blocks), dim3(256), 0, s, c->dev, S, em, seed,
+      hipLaunchKernelGGL(k_emit, dim3(sm ? small_em : em_blocks), dim3(256), 0, s, c->dev, S, em, seed,
                          ctr + C_STORE, shower_energy, ctr + C_STATS);
       HIP_TRY(hipGetLastError());
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * g + 3], s));
+    }
+    return PETITE_OK;
+  };
+  // The capture runs on an internal stream (the caller's may be the null
+  // stream, which cannot capture); the graph is then launched on the caller's.
+  const bool use_graph = tn.graph == 1 && !c->timing;
+  hipGraphExec_t gexec = nullptr;
+  auto drop_graph = [&]() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+  };
+  if (use_graph) {
+    rc = ensure_draw_streams(c);
+    if (rc) return rc;
+    if (!c->cstream) HIP_TRY(hipStreamCreate(&c->cstream));
+  }
+  while (true) {
+    if (use_graph && !small) {
+      if (!gexec) {
+        HIP_TRY(hipStreamBeginCapture(c->cstream, hipStreamCaptureModeRelaxed));
+        const int rq = enqueue_group(false, c->cstream);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->cstream, &graph);
+        if (rq) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return rq;
+        }
+        if (ec != hipSuccess) return fail(PETITE_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+        const hipError_t ei = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) {
+          gexec = nullptr;
+          return fail(PETITE_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+      }
+      const hipError_t el = hipGraphLaunch(gexec, s);
+      if (el != hipSuccess) {
+        drop_graph();
+        return fail(PETITE_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(el));
+      }
+    } else {
+      rc = enqueue_group(small, s);
+      if (rc) {
+        drop_graph();
+        return rc;
+      }
     }
     queued += group;
     HIP_TRY(hipMemcpyAsync(&h_store, ctr + C_STORE, sizeof(h_store), hipMemcpyDeviceToHost, s));
@@ -2217,8 +2271,12 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
     }
     if ((int64_t)h_store > store->capacity) break;  // children were dropped: report the size reached
     if (h_range[1] == 0 && h_range[3] == 0 && (int64_t)h_store == h_range[2]) break;  // the last sweep found nothing to do
-    if (queued > 1000000) return fail(PETITE_ERR_HIP, "petite_run_showers: sweep count runaway");
+    if (queued > 1000000) {
+      drop_graph();
+      return fail(PETITE_ERR_HIP, "petite_run_showers: sweep count runaway");
+    }
   }
+  drop_graph();
   unsigned long long hs[ST_N], sweeps = 0;
   HIP_TRY(hipMemcpyAsync(hs, ctr + C_STATS, sizeof(hs), hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(&sweeps, ctr + C_SWEEPS, sizeof(sweeps), hipMemcpyDeviceToHost, s));

@@ -25,11 +25,11 @@ ME = 0.51099895e-3
 MMU = 0.1056583755
 
 # TINY never takes the small-sweep launches (per-process draw streams
-# throughout); the defaults on these 48 primaries and WIDE take them after the
+# throughout, each 3-sweep group replayed from a captured HIP graph); the defaults on these 48 primaries and WIDE take them after the
 # first host check (every process's draws in one k_draw_small launch; WIDE
 # hands particles on after 2 substeps there)
 TINY = {"batch_lanes": 1, "yield_steps": 3, "transport_waves": 1, "emit_waves": 1, "draw_waves": [1] * 8,
-        "min_group_lanes": 1, "sweep_group": 3, "sweep_bound": 64, "resume_cap": 8, "small_sweep": 1}
+        "min_group_lanes": 1, "sweep_group": 3, "sweep_bound": 64, "resume_cap": 8, "small_sweep": 1, "graph": 1}
 WIDE = {"batch_lanes": 64, "yield_steps": 1 << 30, "transport_waves": 8, "emit_waves": 4,
         "draw_waves": [4, 1, 3, 1, 1, 1, 1, 1], "min_group_lanes": 64, "sweep_group": 1, "small_sweep": 1 << 30,
         "small_yield_steps": 2}
