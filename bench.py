@@ -180,6 +180,7 @@ def bench_c5(n=10_000, reps=2):
                         f"with GPU-trained maps",
             "showers_per_s": n / best, "particle_steps_per_s": st["n_steps"] / best, "ms": best * 1e3,
             "particles": st["n_particles"], "maps": dict(sh._tables.map_source),
+            "sweeps": st["n_iterations"], "tail_launches": st["launches_tail"],
             "draws_by_process": {name: {"draws": d, "trials_per_draw": t / d, "clipped_fraction": v / d}
                                  for name, d, t, v in zip(PROC_NAMES, st["n_draws_proc"], st["n_trials_proc"],
                                                           st["n_viol_proc"]) if d > 0}}
@@ -285,10 +286,11 @@ def main():
         print(f"warmup {w}: {b.n} particles, {b.stats['n_iterations']} sweeps", file=sys.stderr, flush=True)
         del b
     torch.cuda.synchronize()
-    totals = {"steps": 0, "particles": 0, "draws": 0, "trials": 0}
-    # stage times from HIP events on the run's stream: the transport kernel, and
-    # the draws with their emissions (per-process streams, fork to join)
-    kern = {"transport": [0.0, 0], "draw_emit": [0.0, 0]}
+    totals = {"steps": 0, "particles": 0, "draws": 0, "trials": 0, "tail_steps": 0}
+    # stage times from HIP events on the run's stream: the transport kernel,
+    # the draws with their emissions (per-process streams, fork to join), and
+    # the depth-first walk of the last particles
+    kern = {"transport": [0.0, 0], "draw_emit": [0.0, 0], "tail": [0.0, 0]}
     t0 = time.perf_counter()
     for k in range(args.steps):
         b = sh.generate_showers(prim, seed=SEED)
@@ -301,6 +303,9 @@ def main():
         kern["transport"][1] += st["launches_transport"]
         kern["draw_emit"][0] += st["ms_draw"] + st["ms_emit"]
         kern["draw_emit"][1] += st["launches_draw"]
+        kern["tail"][0] += st["ms_tail"]
+        kern["tail"][1] += st["launches_tail"]
+        totals["tail_steps"] += st["n_steps_tail"]
         print(f"step {k}: {b.n} particles, {st['n_steps']} particle-steps", file=sys.stderr, flush=True)
         if k + 1 < args.steps:
             del b
@@ -314,7 +319,7 @@ def main():
     # roofline of the dominant kernel, k_transport (the largest single kernel in
     # the rocprofv3 summary; the draw kernels of the six processes run side by side)
     t_dom = kern["transport"][0] / 1e3
-    alg = B_STEP * totals["steps"]
+    alg = B_STEP * (totals["steps"] - totals["tail_steps"])  # the steps k_transport took
     ach = alg / t_dom / 1e9
     roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
             "traffic": None, "kernel": "k_transport", "per_unit": "128 B per particle-step"}
@@ -385,6 +390,7 @@ def main():
         "showers_per_s": showers_per_s,
         "particles_per_step": totals["particles"] / K,
         "particle_steps_per_step": totals["steps"] / K,
+        "tail_particle_steps_per_step": totals["tail_steps"] / K,
         "trials_per_draw": totals["trials"] / max(totals["draws"], 1),
         "draws_by_process": {name: {"draws": d, "trials_per_draw": t / d, "maxf_violations": v,
                                     "clipped_fraction": v / d}

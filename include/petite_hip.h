@@ -118,6 +118,9 @@ typedef struct {
   int64_t n_viol_proc[PETITE_NPROC];   /* accepted draws with J*f above max_F per process code (the
                                         * accept probability min(1, J*f / max_F) clips them, as the
                                         * reference's test does) */
+  double ms_tail;          /* depth-first walk time (HIP events, with timing enabled) */
+  int64_t launches_tail;   /* depth-first walk launches */
+  int64_t n_steps_tail;    /* the particle-steps of n_steps taken in depth-first walks */
 } PetiteStats;
 
 /* Create a context: copies the table blobs (host pointers) to the device. */
@@ -156,6 +159,13 @@ typedef struct {
                               * is 4 ms per step slower on the 100k lead workload] */
   int32_t graph;             /* 1: a sweep group with per-process draw streams is captured once per
                               * run as a HIP graph and replayed; 2: direct launches [2] */
+  int32_t tail_sweep;        /* once fewer particles than this wait for the next sweep (seen at a host
+                              * check), each remaining particle's subtree is walked depth-first by one
+                              * wave (transport, draw and children in one kernel) [4096]; 1: never */
+  int32_t tail_waves;        /* depth-first walk waves per CU [8] */
+  int32_t tail_depth;        /* depth-first walk: stack entries per wave (1..128); a child past a full
+                              * stack waits for the next launch [128] */
+  int32_t pad;
 } PetiteTuning;
 
 /* Set (tuning != NULL) or reset to the defaults (NULL) the context's launch

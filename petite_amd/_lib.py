@@ -72,7 +72,8 @@ class Stats(C.Structure):
                 ("ms_draw", C.c_double), ("ms_emit", C.c_double), ("launches_transport", C.c_int64),
                 ("launches_draw", C.c_int64), ("launches_emit", C.c_int64),
                 ("n_draws_proc", C.c_int64 * 12), ("n_trials_proc", C.c_int64 * 12),
-                ("n_viol_proc", C.c_int64 * 12)]
+                ("n_viol_proc", C.c_int64 * 12), ("ms_tail", C.c_double), ("launches_tail", C.c_int64),
+                ("n_steps_tail", C.c_int64)]
 
     def as_dict(self):
         return {f: (float(getattr(self, f)) if t is C.c_double else
@@ -86,7 +87,8 @@ class Tuning(C.Structure):
                 ("emit_waves", C.c_int32), ("draw_waves", C.c_int32 * 8), ("min_group_lanes", C.c_int32),
                 ("sweep_group", C.c_int32), ("group_requests", C.c_int32), ("small_sweep", C.c_int32),
                 ("sweep_bound", C.c_int64),
-                ("resume_cap", C.c_int64), ("small_yield_steps", C.c_int32), ("graph", C.c_int32)]
+                ("resume_cap", C.c_int64), ("small_yield_steps", C.c_int32), ("graph", C.c_int32),
+                ("tail_sweep", C.c_int32), ("tail_waves", C.c_int32), ("tail_depth", C.c_int32), ("pad", C.c_int32)]
 
     def as_dict(self):
         return {f: ([int(v) for v in getattr(self, f)] if f == "draw_waves" else int(getattr(self, f)))

@@ -40,6 +40,8 @@ SPACE = {
     "small_sweep": [1, 4096, 16384, 65536, 262144],
     "small_yield_steps": [40, 160, 1 << 30],
     "graph": [1, 2],
+    "tail_sweep": [1, 1024, 4096, 16384, 65536],
+    "tail_waves": [4, 8, 16],
 }
 FIELDS = {f for f, _ in _lib.Tuning._fields_ if f != "pad"}
 

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libpetite_hip.so")
 SOURCES = [os.path.join(HERE, "csrc", "petite_hip.hip")]
-HEADERS = ["philox.h", "physics.h", "transport.h", "sweep.h", "quad.h", "dark.h"]
+HEADERS = ["philox.h", "physics.h", "transport.h", "sweep.h", "tail.h", "quad.h", "dark.h"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -55,6 +55,8 @@ enum CounterIdx {
   C_MESON_N = 24,             // meson-decay entries written in the current pass
   C_YIELD_OUT = 26,           // particles the current sweep's transport handed on to the next sweep
   C_RPARITY = 27,             // which half of the resume buffer the current sweep reads
+  C_SPILL = 28,               // [2] depth-first walk: entries in each spill list
+  C_TAIL_STEPS = 30,          // particle-steps taken in depth-first walks
   C_STATS = 32,               // [ST_N] run statistics
   C_PTRIALS = 64,             // [PETITE_NPROC] candidates per process
   C_PDRAWS = 80,              // [PETITE_NPROC] draws per process
@@ -1324,6 +1326,7 @@ __global__ void k_sweep_setup(unsigned long long* ctr, int64_t* range, unsigned 
   }
 }
 
+#include "tail.h"
 #include "quad.h"
 #include "dark.h"
 
@@ -1495,6 +1498,7 @@ struct PetiteCtx {
   int64_t* drange = nullptr;
   ResumeRec* dresume = nullptr;  // [2][resume_cap]
   int64_t resume_cap = 0;
+  int32_t* tail_spill = nullptr;  // [2][kTailSpill]: depth-first walk spill lists
   std::vector<hipEvent_t> tev;
   int n_cu = 256;
   PetiteTuning tune{};  // launch and queue sizes in effect (defaults filled in)
@@ -1577,6 +1581,9 @@ static PetiteTuning default_tuning() {
   t.small_sweep = 1 << 20;
   t.small_yield_steps = 40;
   t.graph = 2;
+  t.tail_sweep = 4096;
+  t.tail_waves = 8;
+  t.tail_depth = kTailMaxDepth;
   t.sweep_bound = (int64_t)1 << 23;
   t.resume_cap = (int64_t)1 << 20;
   return t;
@@ -1871,7 +1878,7 @@ void petite_ctx_destroy(PetiteCtx* c) {
                   (void*)c->res_trials, (void*)c->res_w, (void*)c->req_count, (void*)c->counters, (void*)c->ddesc,
                   (void*)c->dsig_lds, (void*)c->dark_blob, (void*)c->dyield, (void*)c->yield_blob, (void*)c->yacc, (void*)c->ytab_dev,
                   (void*)c->bnd_blob, (void*)c->bnd_tab, (void*)c->bnd_slots, (void*)c->bnd_jobs, (void*)c->bnd_bad,
-                  (void*)c->par.sid, (void*)c->par.pf, (void*)c->par.pos, (void*)c->par.info})
+                  (void*)c->par.sid, (void*)c->par.pf, (void*)c->par.pos, (void*)c->par.info, (void*)c->tail_spill})
     if (p) (void)hipFree(p);
   free_entries(c);
   delete c;
@@ -1899,6 +1906,12 @@ int petite_set_tuning(PetiteCtx* c, const PetiteTuning* in) {
   t.small_sweep = pick(t.small_sweep, d.small_sweep);
   t.small_yield_steps = pick(t.small_yield_steps, d.small_yield_steps);
   t.graph = pick(t.graph, d.graph);
+  t.tail_sweep = pick(t.tail_sweep, d.tail_sweep);
+  t.tail_waves = pick(t.tail_waves, d.tail_waves);
+  t.tail_depth = pick(t.tail_depth, d.tail_depth);
+  t.pad = 0;
+  if (t.tail_waves > 64) return fail(PETITE_ERR_ARG, "petite_set_tuning: more than 64 tail waves per CU");
+  if (t.tail_depth > kTailMaxDepth) return fail(PETITE_ERR_ARG, "petite_set_tuning: tail_depth above 128");
   if (t.graph > 2) return fail(PETITE_ERR_ARG, "petite_set_tuning: graph is 1 (capture) or 2 (direct launches)");
   if (t.batch_lanes > 64) return fail(PETITE_ERR_ARG, "petite_set_tuning: batch_lanes above 64");
   if (t.transport_waves > 64 || t.emit_waves > 64) return fail(PETITE_ERR_ARG, "petite_set_tuning: more than 64 waves per CU");
@@ -2159,8 +2172,16 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   for (int p = 0; p < 8; ++p)
     if (c->desc.grid[p].n_E > 0 && (muons || (p != MUONE && p != MUONBREM))) small_procs |= 1u << p;
   const size_t lds = (size_t)c->dev.lds_doubles * sizeof(double);
+  // the depth-first walk of the last particles (k_tail): entered once fewer
+  // than tn.tail_sweep particles wait at a host check, then one launch per
+  // check until no spill is left
+  bool tail = false;
+  int tpar = 0;  // spill list the next walk reads
+  int64_t tail_launches = 0;
+  double t_tail = 0.0;
+  const unsigned tail_blocks = blocks_for(c, tn.tail_waves);
   if (c->timing) {
-    while ((int)c->tev.size() < 4 * group) {
+    while ((int)c->tev.size() < 4 * group + 2) {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
       c->tev.push_back(e);
@@ -2216,6 +2237,36 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
     if (!c->cstream) HIP_TRY(hipStreamCreate(&c->cstream));
   }
   while (true) {
+    if (tail) {
+      if (!c->tail_spill) HIP_TRY(hipMalloc(&c->tail_spill, 2 * kTailSpill * sizeof(int32_t)));
+      hipLaunchKernelGGL(k_tail_setup, dim3(1), dim3(64), 0, s, ctr, c->drange, store->capacity, rcap, tpar);
+      HIP_TRY(hipGetLastError());
+      if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * group], s));
+      const TailParams tp{c->tail_spill + (int64_t)tpar * kTailSpill, ctr + C_SPILL + tpar,
+                          c->tail_spill + (int64_t)(tpar ^ 1) * kTailSpill, ctr + C_SPILL + (tpar ^ 1), kTailSpill,
+                          tn.tail_depth};
+      hipLaunchKernelGGL(k_tail, dim3(tail_blocks), dim3(256), lds, s, c->dev, S, seed, shower_energy, ctr,
+                         c->dsig_lds, (const int64_t*)c->drange, ResumeBuf{c->dresume, rcap}, tp);
+      HIP_TRY(hipGetLastError());
+      if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * group + 1], s));
+      hipLaunchKernelGGL(k_tail_finish, dim3(1), dim3(64), 0, s, ctr, c->drange, store->capacity);
+      HIP_TRY(hipGetLastError());
+      tpar ^= 1;
+      ++tail_launches;
+      unsigned long long h_spill = 0;
+      HIP_TRY(hipMemcpyAsync(&h_store, ctr + C_STORE, sizeof(h_store), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(&h_spill, ctr + C_SPILL + tpar, sizeof(h_spill), hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      if (c->timing) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, c->tev[4 * group], c->tev[4 * group + 1]));
+        t_tail += ms;
+      }
+      if ((int64_t)h_store > store->capacity) break;
+      if (h_spill == 0) break;  // every particle walked
+      if (tail_launches > 100000) return fail(PETITE_ERR_HIP, "petite_run_showers: tail launch runaway");
+      continue;
+    }
     if (use_graph && !small) {
       if (!gexec) {
         HIP_TRY(hipStreamBeginCapture(c->cstream, hipStreamCaptureModeRelaxed));
@@ -2247,7 +2298,9 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
       }
     }
     queued += group;
+    unsigned long long h_yield = 0;
     HIP_TRY(hipMemcpyAsync(&h_store, ctr + C_STORE, sizeof(h_store), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&h_yield, ctr + C_YIELD_OUT, sizeof(h_yield), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(h_range, c->drange, sizeof(h_range), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (c->timing) {
@@ -2271,6 +2324,14 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
     }
     if ((int64_t)h_store > store->capacity) break;  // children were dropped: report the size reached
     if (h_range[1] == 0 && h_range[3] == 0 && (int64_t)h_store == h_range[2]) break;  // the last sweep found nothing to do
+    {
+      // particles waiting for the next sweep: children of the last one and its hand-ons
+      const int64_t waiting = (int64_t)h_store - h_range[2] + std::min<int64_t>((int64_t)h_yield, rcap);
+      if (waiting > 0 && waiting < (int64_t)tn.tail_sweep) {
+        drop_graph();
+        tail = true;
+      }
+    }
     if (queued > 1000000) {
       drop_graph();
       return fail(PETITE_ERR_HIP, "petite_run_showers: sweep count runaway");
@@ -2295,7 +2356,10 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
     HIP_TRY(hipMemcpyAsync(pt, ctr + C_PTRIALS, sizeof(pt), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(pd, ctr + C_PDRAWS, sizeof(pd), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(pv, ctr + C_PVIOL, sizeof(pv), hipMemcpyDeviceToHost, s));
+    unsigned long long ts = 0;
+    HIP_TRY(hipMemcpyAsync(&ts, ctr + C_TAIL_STEPS, sizeof(ts), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
+    stats->n_steps_tail = (int64_t)ts;
     for (int p = 0; p < PETITE_NPROC; ++p) {
       stats->n_viol_proc[p] = (int64_t)pv[p];
       stats->n_trials_proc[p] = (int64_t)pt[p];
@@ -2310,6 +2374,9 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   stats->launches_transport = queued;
   stats->launches_draw = queued;
   stats->launches_emit = queued;
+  stats->ms_tail = t_tail;
+  stats->launches_tail = tail_launches;
+
   if (hs[ST_REQ_OVERFLOW])
     return fail(PETITE_ERR_INTERNAL, "petite_run_showers: draw requests overflowed their region (library bug)");
   if (written > store->capacity)

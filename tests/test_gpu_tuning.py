@@ -6,7 +6,11 @@ stream id), bit for bit.  The small settings below also drive the paths the
 default sizes rarely reach on small inputs: sweeps limited by their bound
 with hand-ons pending (the clamp of new particles to bound - resumed), a
 resume buffer that fills up (the lane keeps stepping), single-lane refill
-batches, one-wave launches and single-lane draw groups.
+batches, one-wave launches and single-lane draw groups.  The defaults walk
+these 48 showers depth-first after the first host check (k_tail); TINY and
+WIDE never do, so they hold the walk to the sweeps' records; DEEP walks from
+the second sweep with a one-entry stack, so most children go through the
+spill lists and several launches, hand-ons of the first sweep included.
 """
 import math
 
@@ -29,10 +33,12 @@ MMU = 0.1056583755
 # first host check (every process's draws in one k_draw_small launch; WIDE
 # hands particles on after 2 substeps there)
 TINY = {"batch_lanes": 1, "yield_steps": 3, "transport_waves": 1, "emit_waves": 1, "draw_waves": [1] * 8,
-        "min_group_lanes": 1, "sweep_group": 3, "sweep_bound": 64, "resume_cap": 8, "small_sweep": 1, "graph": 1}
+        "min_group_lanes": 1, "sweep_group": 3, "sweep_bound": 64, "resume_cap": 8, "small_sweep": 1, "graph": 1,
+        "tail_sweep": 1}
 WIDE = {"batch_lanes": 64, "yield_steps": 1 << 30, "transport_waves": 8, "emit_waves": 4,
         "draw_waves": [4, 1, 3, 1, 1, 1, 1, 1], "min_group_lanes": 64, "sweep_group": 1, "small_sweep": 1 << 30,
-        "small_yield_steps": 2}
+        "small_yield_steps": 2, "tail_sweep": 1}
+DEEP = {"yield_steps": 2, "sweep_group": 1, "small_sweep": 1, "tail_sweep": 1 << 30, "tail_waves": 1, "tail_depth": 1}
 
 
 def _primaries(n, seed):
@@ -72,7 +78,7 @@ def reference_run(lead):
     return prim, batch
 
 
-@pytest.mark.parametrize("name,tuning", [("tiny", TINY), ("wide", WIDE)])
+@pytest.mark.parametrize("name,tuning", [("tiny", TINY), ("wide", WIDE), ("deep", DEEP)])
 def test_records_do_not_depend_on_tuning(lead, reference_run, name, tuning):
     prim, ref = reference_run
     ctx = lead.device_context()
@@ -92,6 +98,12 @@ def test_records_do_not_depend_on_tuning(lead, reference_run, name, tuning):
     if name == "tiny":
         # the small bound splits the run into many more sweeps
         assert batch.stats["n_iterations"] > ref.stats["n_iterations"]
+    if name in ("tiny", "wide"):
+        assert batch.stats["launches_tail"] == 0
+    else:
+        # one-entry stacks: the spilled children take further launches
+        assert batch.stats["launches_tail"] > 1
+    assert ref.stats["launches_tail"] >= 1
     assert batch.stats["n_steps"] == ref.stats["n_steps"]
     assert batch.stats["n_trials"] == ref.stats["n_trials"]
 
@@ -132,9 +144,11 @@ def test_tuning_validation(lead):
     from petite_amd._lib import PetiteError
     ctx = lead.device_context()
     for bad in ({"batch_lanes": 65}, {"min_group_lanes": 3}, {"min_group_lanes": 128}, {"sweep_bound": 32}, {"sweep_group": 65},
-                {"sweep_bound": 128, "resume_cap": 256}, {"transport_waves": 65}):
+                {"sweep_bound": 128, "resume_cap": 256}, {"transport_waves": 65}, {"tail_depth": 129},
+                {"tail_waves": 65}):
         with pytest.raises(PetiteError):
             ctx.set_tuning(bad)
     d = ctx.set_tuning(None)
     assert d["batch_lanes"] == 44 and d["yield_steps"] == 40 and d["sweep_bound"] == 1 << 23
     assert d["resume_cap"] == 1 << 20 and d["draw_waves"][0] == 16
+    assert d["tail_sweep"] == 4096 and d["tail_waves"] == 8 and d["tail_depth"] == 128
