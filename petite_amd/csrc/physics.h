@@ -177,35 +177,43 @@ __device__ inline float cospi_f(double t) {
   return sinpi_f(0.5 - t);
 }
 
+// 1/x by the hardware reciprocal (1 ulp).  The float32 integrand bodies below
+// only decide candidates outside the band kRefBand (4e-4 relative) around the
+// accept threshold, inside which the float64 reference formula decides; their
+// error against it (a few 1e-6) leaves the decisions unchanged, and the
+// correctly rounded division costs several times the instructions.
+__device__ inline float rcp_f(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Brem / MuonBrem, all_processes.py:139-206.  PF*FF = C epp d dp /(w ep (1+a0^2 q^2)^2).
 __device__ inline float f_brem(const DevTables& T, double ep, double m, float C, const double* x) {
   const double egmin = T.d->Eg_min;
   const double range = ep - m - egmin;
   const double w = egmin + x[0] * range;
   const double epp = ep - w;
-  const double k = ep / (2.0 * m);
+  const double k = ep * (0.5 / m);
   const double d = k * (x[1] + x[2]);
   const double dp = k * (x[1] - x[2]);
   if (!((egmin < w) && (w < ep - m) && (m < epp) && (epp < ep) && (d > 0.0) && (dp > 0.0))) return 0.0f;
   const float fd = (float)d, fdp = (float)dp, fw = (float)w, fepp = (float)epp, fep = (float)ep, fm = (float)m;
-  const float dmd = (float)(ep / m * x[2]);                  // d - dp
+  const float dmd = (float)(2.0 * k * x[2]);                 // d - dp
   const float Bn = (float)(-w + d * d * epp - dp * dp * ep);  // (1+d^2) epp - (1+dp^2) ep
   const float s2 = sinpi_f(x[3] - 0.5);                      // sin(ph/2)
-  const float B = Bn / (2.0f * fep * fepp);
+  const float r_pp = rcp_f(fep * fepp);                      // 1/(ep epp)
+  const float B = 0.5f * Bn * r_pp;
   const float qm2 = dmd * dmd + 4.0f * fd * fdp * s2 * s2 + fm * fm * B * B;
   const float qsq = fm * fm * qm2;
-  const float den = 1.0f + T.a0sq * qsq;
-  const float pfff = C * fepp * fd * fdp / (fw * fep * den * den);
-  const float jac = (float)kPi * fep * fep * (float)range / (fm * fm);
+  const float rden = rcp_f(1.0f + T.a0sq * qsq);
+  const float pfff = C * fepp * fd * fdp * rcp_f(fw * fep) * rden * rden;
+  const float jac = (float)kPi * fep * fep * (float)range * (float)(1.0 / (m * m));
   // T1+T2+T3+T4 of the reference as a sum of non-negative terms: the direct
   // sum cancels to O((w/ep)^2) for soft photons.  With a=d, b=dp, A=1/(1+a^2),
   // B=1/(1+b^2):  (a-b)^2 (1-ab)^2 A^2 B^2 + 4 a b sin^2(ph/2) A B
   //               + w^2/(2 ep epp) A B ((a-b)^2 + 4 a b sin^2(ph/2))
-  const float i1 = 1.0f / (1.0f + fd * fd), i2 = 1.0f / (1.0f + fdp * fdp);
+  const float i12 = rcp_f((1.0f + fd * fd) * (1.0f + fdp * fdp));
   const float ab = fd * fdp;
-  const float q1 = dmd * (1.0f - ab) * i1 * i2;
+  const float q1 = dmd * (1.0f - ab) * i12;
   const float ang = 4.0f * ab * s2 * s2;
-  const float bracket = q1 * q1 + ang * i1 * i2 + fw * fw / (2.0f * fep * fepp) * i1 * i2 * (dmd * dmd + ang);
+  const float bracket = q1 * q1 + ang * i12 + 0.5f * fw * fw * r_pp * i12 * (dmd * dmd + ang);
   return pfff * bracket * jac;
 }
 
@@ -215,24 +223,25 @@ __device__ inline float f_pairprod(const DevTables& T, double w, const double* x
   const double r2 = w - 2.0 * m;
   const double epp = m + x[0] * r2;
   const double epm = m + (1.0 - x[0]) * r2;
-  const double k = w / (2.0 * m);
+  const double k = w * (0.5 / m);
   const double dp = k * (x[1] + x[2]);
   const double dm = k * (x[1] - x[2]);
   if (!((m < epm) && (epm < w) && (m < epp) && (epp < w) && (dm > 0.0) && (dp > 0.0))) return 0.0f;
   const float fdp = (float)dp, fdm = (float)dm, fw = (float)w, fepp = (float)epp, fepm = (float)epm, fm = (float)m;
-  const float dpm = (float)(w / m * x[2]);                   // dp - dm
+  const float dpm = (float)(2.0 * k * x[2]);                 // dp - dm
   const float c2 = cospi_f(x[3]);                            // cos(ph/2)
   const float dp2 = fdp * fdp, dm2 = fdm * fdm;
-  const float S = (1.0f + dp2) / (2.0f * fepp) + (1.0f + dm2) / (2.0f * fepm);
+  const float r_pm = rcp_f(fepp * fepm);                     // 1/(epp epm)
+  const float S = 0.5f * ((1.0f + dp2) * fepm + (1.0f + dm2) * fepp) * r_pm;
   const float qq = dpm * dpm + 4.0f * fdp * fdm * c2 * c2 + fm * fm * S * S;
-  const float den = 1.0f + T.a0sq * fm * fm * qq;
-  const float pfff = T.brem_C_e * fepp * fepm * fdp * fdm / (fw * fw * fw * den * den);
-  const float jac = (float)kPi * fw * fw * (float)r2 / (fm * fm);
+  const float rden = rcp_f(1.0f + T.a0sq * fm * fm * qq);
+  const float pfff = T.brem_C_e * fepp * fepm * fdp * fdm * rcp_f(fw * fw * fw) * rden * rden;
+  const float jac = (float)kPi * fw * fw * (float)r2 * (float)(1.0 / (m * m));
   // T1+T2+T3+T4 of the reference as a sum of non-negative terms (a=dp, b=dm):
   //   (a^2-b^2)^2 A^2 B^2 + (epp^2+epm^2)/(2 epp epm) A B ((a-b)^2 + 4 a b cos^2(ph/2))
-  const float i1 = 1.0f / (1.0f + dp2), i2 = 1.0f / (1.0f + dm2);
-  const float t1 = dpm * (fdp + fdm) * i1 * i2;
-  const float bracket = t1 * t1 + (fepp * fepp + fepm * fepm) / (2.0f * fepp * fepm) * i1 * i2 *
+  const float i12 = rcp_f((1.0f + dp2) * (1.0f + dm2));
+  const float t1 = dpm * (fdp + fdm) * i12;
+  const float bracket = t1 * t1 + 0.5f * (fepp * fepp + fepm * fepm) * r_pm * i12 *
                                       (dpm * dpm + 4.0f * fdp * fdm * c2 * c2);
   return pfff * bracket * jac;
 }
