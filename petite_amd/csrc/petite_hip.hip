@@ -729,9 +729,12 @@ struct LdsSigma {
 // azimuth word (moliere.py:265-284, 350-400).
 __device__ inline void mcs_rotate(const DevTables& T, float& nx, float& ny, float& nz, double E, double p, double m_part,
                                   double m_lep, double dz, uint32_t w_r, uint32_t w_a) {
-  const float beta = (float)(p / E);
-  const float pMeV = (float)(m_lep * 1.0e3 * (p / m_part));
-  const float t = (float)(T.d->rho * (dz / kCmToM));
+  // float32 quotients by the hardware reciprocal: the angle is a float32
+  // quantity (a float64 division per factor costs several times as much)
+  const float fp = (float)p;
+  const float beta = fp * __builtin_amdgcn_rcpf((float)E);
+  const float pMeV = (float)(m_lep * 1.0e3) * fp * __builtin_amdgcn_rcpf((float)m_part);
+  const float t = (float)(T.d->rho * (dz * (1.0 / kCmToM)));
   const float th0 = lynch_dahl_theta0(T.mcs_cA, T.mcs_cB, T.mcs_cC, t, pMeV, beta);
   const float r = sqrtf(-2.0f * logf(u01f(w_r)));
   const float theta = ((w_a & 1u) ? r : -r) * th0 * (float)T.d->rescale_MCS;
@@ -1032,7 +1035,7 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
         // hard if u > exp(-dz/mfp): the float32 exponential decides unless u
         // lies within 1e-5 of it (its error is a few 1e-7), where float64 does
         const double ux = u01d(u.x);
-        double thr = (double)__expf(-1.0f / (float)div);
+        double thr = (double)__expf(-__builtin_amdgcn_rcpf((float)div));
         if (fabs(ux - thr) < 1.0e-5) thr = exp(-(dz / mfp));
         if (ux > thr) {
           adv = false;

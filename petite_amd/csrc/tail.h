@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_tail(DevTables T, DevStore S, uint64_t 
         w_r = u.z;
         w_a = u.w;
         const double ux = u01d(u.x);
-        double thr = (double)__expf(-1.0f / (float)div);
+        double thr = (double)__expf(-__builtin_amdgcn_rcpf((float)div));
         if (fabs(ux - thr) < 1.0e-5) thr = exp(-(dz / mfp));
         if (ux > thr) {
           adv = false;
