@@ -727,11 +727,10 @@ struct LdsSigma {
 // Lynch-Dahl scattering of the unit direction (nx,ny,nz) over ``dz`` metres:
 // radius and azimuth from two Philox words, the sign from bit 0 of the
 // azimuth word (moliere.py:265-284, 350-400).
-__device__ inline void mcs_rotate(const DevTables& T, float& nx, float& ny, float& nz, double E, double p, double m_part,
+__device__ inline void mcs_rotate(const DevTables& T, float& nx, float& ny, float& nz, double E, float fp, double m_part,
                                   double m_lep, double dz, uint32_t w_r, uint32_t w_a) {
   // float32 quotients by the hardware reciprocal: the angle is a float32
   // quantity (a float64 division per factor costs several times as much)
-  const float fp = (float)p;
   const float beta = fp * __builtin_amdgcn_rcpf((float)E);
   const float pMeV = (float)(m_lep * 1.0e3) * fp * __builtin_amdgcn_rcpf((float)m_part);
   const float t = (float)(T.d->rho * (dz * (1.0 / kCmToM)));
@@ -752,23 +751,28 @@ __device__ inline void mcs_rotate(const DevTables& T, float& nx, float& ny, floa
   nz = oz * inv;
 }
 
+// |p| of energy E and mass m.  fmax: contracted to fma(E, E, -(m*m)), E*E -
+// m*m at E == m is minus the rounding error of m*m, which can be negative (the
+// reference's p is 0 there).
+__device__ inline double momentum(double E, double m) { return sqrt(fmax(E * E - m * m, 0.0)); }
+
 // Moves a charged particle by ``dist`` after losing dEdx*dist (lose_energy,
-// particle.py:143-153) and scatters it (shower.py:569-581, 589-598).
+// particle.py:143-153) and scatters it (shower.py:569-581, 589-598).  The
+// float64 |p| is formed once, when the particle finishes (momentum()); the
+// scattering angle takes it in float32.
 __device__ inline void charged_advance(const DevTables& T, double dist, double m, double m_lep, uint32_t w_r,
-                                       uint32_t w_a, double& E, double& p, float& nx, float& ny, float& nz, float& rx,
+                                       uint32_t w_a, double& E, float& nx, float& ny, float& nz, float& rx,
                                        float& ry, float& rz, double& dep) {
   const double Eb = E;
   E = E - T.d->dEdx * dist;
   if (E <= m) E = m;
-  // fmax: contracted to fma(E, E, -(m*m)), E*E - m*m at E == m is minus the
-  // rounding error of m*m, which can be negative (the reference's p is 0 there)
-  p = sqrt(fmax(E * E - m * m, 0.0));
+  const double q = fmax(E * E - m * m, 0.0);  // p^2, as momentum() forms it
   dep += Eb - E;
-  if (p > 0.0) {
+  if (q > 0.0) {
     rx += (float)(nx * dist);
     ry += (float)(ny * dist);
     rz += (float)(nz * dist);
-    if (T.d->MS) mcs_rotate(T, nx, ny, nz, E, p, m, m_lep, dist, w_r, w_a);
+    if (T.d->MS) mcs_rotate(T, nx, ny, nz, E, sqrtf((float)q), m, m_lep, dist, w_r, w_a);
   } else {
     nx = ny = nz = 0.0f;
   }
@@ -871,7 +875,6 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
           ry = R0.ry;
           rz = R0.rz;
           k = R0.k;
-          p = sqrt(fmax(E * E - m * m, 0.0));
           kl = 0;
           closing = false;
           act = true;
@@ -902,7 +905,7 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
           // a species the shower does not propagate (mesons, neutrinos: no
           // channels and not a photon) keeps its |p|, and ends where it starts
           const bool still = ch == 0xFFFu && pid != 22;
-          p = still ? sqrt((double)pn2) : sqrt(fmax(E * E - m * m, 0.0));
+          p = still ? sqrt((double)pn2) : 0.0;  // the others' |p| is formed when they finish
           rx = r0.x;
           ry = r0.y;
           rz = r0.z;
@@ -934,6 +937,9 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
       int want = -1;
       if (wait) {
         steps += k;
+        // |p| at the final energy (a species the shower does not propagate
+        // keeps the |p| it came with)
+        if (!(ch == 0xFFFu && pid != 22)) p = momentum(E, m);
         S.pf[slot] = make_float4((float)E, (float)(p * nx), (float)(p * ny), (float)(p * nz));
         S.rf[slot] = make_float4(rx, ry, rz, (float)k);
         // process choice (shower.py:665-698) behind sample_scattering's threshold (469-470)
@@ -1054,7 +1060,7 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
         w_a = u.z;
         m_lep = kMe;
       }
-      if (adv) charged_advance(T, dist, m, m_lep, w_r, w_a, E, p, nx, ny, nz, rx, ry, rz, dep);
+      if (adv) charged_advance(T, dist, m, m_lep, w_r, w_a, E, nx, ny, nz, rx, ry, rz, dep);
       ++k;
       if (in_close) {
         closing = false;

@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void k_tail(DevTables T, DevStore S, uint64_t 
     const uint32_t ch = sum_channels(pid);
     const double m = pid_mass(pid);
     const double pme = fmax(fmax(min_calc(T, pid), T.d->min_energy), m);
-    double E, p, dep = 0.0, dz = 0.0;
+    double E, p = 0.0, dep = 0.0, dz = 0.0;
     float nx, ny, nz, rx, ry, rz;
     uint32_t k = 0;
     bool act = false;
@@ -167,7 +167,6 @@ __global__ __launch_bounds__(256) void k_tail(DevTables T, DevStore S, uint64_t 
       ry = R0.ry;
       rz = R0.rz;
       k = R0.k;
-      p = sqrt(fmax(E * E - m * m, 0.0));
       act = true;
     } else {
       const float4 p0 = S.p0[slot];
@@ -184,7 +183,7 @@ __global__ __launch_bounds__(256) void k_tail(DevTables T, DevStore S, uint64_t 
         nz *= inv;
       }
       const bool still = ch == 0xFFFu && pid != 22;
-      p = still ? sqrt((double)pn2) : sqrt(fmax(E * E - m * m, 0.0));
+      p = still ? sqrt((double)pn2) : 0.0;  // the others' |p| is formed when they finish
       rx = r0.x;
       ry = r0.y;
       rz = r0.z;
@@ -238,13 +237,14 @@ __global__ __launch_bounds__(256) void k_tail(DevTables T, DevStore S, uint64_t 
         w_a = u.z;
         m_lep = kMe;
       }
-      if (adv) charged_advance(T, dist, m, m_lep, w_r, w_a, E, p, nx, ny, nz, rx, ry, rz, dep);
+      if (adv) charged_advance(T, dist, m, m_lep, w_r, w_a, E, nx, ny, nz, rx, ry, rz, dep);
       ++k;
       if (in_close) act = false;
       else if (E < pme) closing = true;
     }
     // ---- the final state and the process choice (k_transport's finish)
     steps += k;
+    if (!(ch == 0xFFFu && pid != 22)) p = momentum(E, m);
     const float4 pf = make_float4((float)E, (float)(p * nx), (float)(p * ny), (float)(p * nz));
     if (lane == 0) {
       S.pf[slot] = pf;
