@@ -250,8 +250,13 @@ class DeviceContext:
         prim_t = torch.from_numpy(prim.view(np.uint8).copy()).to(dev)
         n_showers = int(prim["shower"].max()) + 1 if n else 0
         if capacity is None:
+            # records per unit of E / cutoff: an electromagnetic shower leaves
+            # about 1.5 (100k lead: 1.5, 10 GeV graphite: 0.7), a muon with its
+            # knock-on electrons' showers about 4 (400 GeV in lead); a store
+            # that fills up costs a second run
             e = np.maximum(prim["E"], 0.0)
-            capacity = int(np.sum(2.5 * e / max(self.desc.min_energy, 1e-3)) + 64 * n + 1024)
+            per = np.where(np.abs(prim["pid"]) == 13, 5.0, 2.5)
+            capacity = int(np.sum(per * e / max(self.desc.min_energy, 1e-3)) + 64 * n + 1024)
         capacity = int(min(capacity, 2 ** 31 - 1))
         while True:
             store = {
