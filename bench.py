@@ -41,6 +41,20 @@ HBM_PEAK_GBS = 8000.0
 B_STEP = 128
 
 
+def measured_valu(kernel):
+    """The instruction roofline of `kernel` from the committed rocprofv3 SQ
+    counter pass of this workload (profiles/valu.json, made by the command it
+    records): VALU issue as a fraction of the SIMDs' issue peak, SIMD VALU-busy
+    fraction and resident waves per SIMD."""
+    path = os.path.join(ROOT, "profiles", "valu.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)[kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return dict(t, source="profiles/valu.json")
+
+
 def measured_traffic(kernel, alg_bytes_per_launch):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE measurement of this workload (profiles/traffic.json, made by the
@@ -325,6 +339,9 @@ def main():
             "traffic": None, "kernel": "k_transport", "per_unit": "128 B per particle-step"}
     roof["avg_launch_ms"] = kern["transport"][0] / max(kern["transport"][1], 1)
     roof["traffic"] = measured_traffic(roof["kernel"], alg / max(kern["transport"][1], 1))
+    # the bound that holds: k_transport is VALU-issue bound (its HBM traffic is
+    # a fifth of the algorithmic bytes: a particle stays in registers)
+    roof["valu"] = measured_valu(roof["kernel"])
     kernels = {k: {"ms_total": v[0] / K, "launches": v[1] // K} for k, v in kern.items()}
 
     cpu = None
