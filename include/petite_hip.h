@@ -161,11 +161,13 @@ typedef struct {
                               * run as a HIP graph and replayed; 2: direct launches [2] */
   int32_t tail_sweep;        /* once fewer particles than this wait for the next sweep (seen at a host
                               * check), each remaining particle's subtree is walked depth-first by one
-                              * wave (transport, draw and children in one kernel) [4096]; 1: never */
+                              * wave (transport, draw and children in one kernel) [16384]; 1: never */
   int32_t tail_waves;        /* depth-first walk waves per CU [8] */
   int32_t tail_depth;        /* depth-first walk: stack entries per wave (1..128); a child past a full
                               * stack waits for the next launch [128] */
-  int32_t pad;
+  int32_t tail_budget;       /* depth-first walk: particles a wave walks from one claimed particle before
+                              * it leaves the rest of its stack to the next launch [16]; a value above
+                              * 2^30 never leaves it */
 } PetiteTuning;
 
 /* Set (tuning != NULL) or reset to the defaults (NULL) the context's launch

@@ -88,7 +88,8 @@ class Tuning(C.Structure):
                 ("sweep_group", C.c_int32), ("group_requests", C.c_int32), ("small_sweep", C.c_int32),
                 ("sweep_bound", C.c_int64),
                 ("resume_cap", C.c_int64), ("small_yield_steps", C.c_int32), ("graph", C.c_int32),
-                ("tail_sweep", C.c_int32), ("tail_waves", C.c_int32), ("tail_depth", C.c_int32), ("pad", C.c_int32)]
+                ("tail_sweep", C.c_int32), ("tail_waves", C.c_int32), ("tail_depth", C.c_int32),
+                ("tail_budget", C.c_int32)]
 
     def as_dict(self):
         return {f: ([int(v) for v in getattr(self, f)] if f == "draw_waves" else int(getattr(self, f)))

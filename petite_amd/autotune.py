@@ -42,6 +42,7 @@ SPACE = {
     "graph": [1, 2],
     "tail_sweep": [1, 1024, 4096, 16384, 65536],
     "tail_waves": [4, 8, 16],
+    "tail_budget": [8, 16, 64, 1 << 30],
 }
 FIELDS = {f for f, _ in _lib.Tuning._fields_ if f != "pad"}
 

@@ -1590,9 +1590,10 @@ static PetiteTuning default_tuning() {
   t.small_sweep = 1 << 20;
   t.small_yield_steps = 40;
   t.graph = 2;
-  t.tail_sweep = 4096;
+  t.tail_sweep = 16384;
   t.tail_waves = 8;
   t.tail_depth = kTailMaxDepth;
+  t.tail_budget = 16;
   t.sweep_bound = (int64_t)1 << 23;
   t.resume_cap = (int64_t)1 << 20;
   return t;
@@ -1918,7 +1919,7 @@ int petite_set_tuning(PetiteCtx* c, const PetiteTuning* in) {
   t.tail_sweep = pick(t.tail_sweep, d.tail_sweep);
   t.tail_waves = pick(t.tail_waves, d.tail_waves);
   t.tail_depth = pick(t.tail_depth, d.tail_depth);
-  t.pad = 0;
+  t.tail_budget = pick(t.tail_budget, d.tail_budget);
   if (t.tail_waves > 64) return fail(PETITE_ERR_ARG, "petite_set_tuning: more than 64 tail waves per CU");
   if (t.tail_depth > kTailMaxDepth) return fail(PETITE_ERR_ARG, "petite_set_tuning: tail_depth above 128");
   if (t.graph > 2) return fail(PETITE_ERR_ARG, "petite_set_tuning: graph is 1 (capture) or 2 (direct launches)");
@@ -2253,7 +2254,7 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
       if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * group], s));
       const TailParams tp{c->tail_spill + (int64_t)tpar * kTailSpill, ctr + C_SPILL + tpar,
                           c->tail_spill + (int64_t)(tpar ^ 1) * kTailSpill, ctr + C_SPILL + (tpar ^ 1), kTailSpill,
-                          tn.tail_depth};
+                          tn.tail_depth, tn.tail_budget};
       hipLaunchKernelGGL(k_tail, dim3(tail_blocks), dim3(256), lds, s, c->dev, S, seed, shower_energy, ctr,
                          c->dsig_lds, (const int64_t*)c->drange, ResumeBuf{c->dresume, rcap}, tp);
       HIP_TRY(hipGetLastError());

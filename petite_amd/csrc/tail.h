@@ -33,7 +33,8 @@ struct TailParams {
   int32_t* spill_out;                 // slots past a full stack
   unsigned long long* n_spill_out;
   int64_t spill_cap;
-  int32_t depth;  // stack entries per wave (1..kTailMaxDepth)
+  int32_t depth;   // stack entries per wave (1..kTailMaxDepth)
+  int32_t budget;  // particles a wave walks from one claimed particle before it passes its stack on
 };
 
 // The wave's stack in registers: entry e sits in lane e % 64, in `lo` for
@@ -126,12 +127,27 @@ __global__ __launch_bounds__(256) void k_tail(DevTables T, DevStore S, uint64_t 
   const int depth = TPm.depth < 1 ? 1 : (TPm.depth > kTailMaxDepth ? kTailMaxDepth : TPm.depth);
   WaveStack st;
   unsigned long long steps = 0, n_trials = 0, n_draws = 0, n_viol = 0, n_fail = 0, n_drop = 0, n_ovf = 0, n_lost = 0;
+  int walked = 0;  // particles walked since the wave's last claim
   while (true) {
+    // a claimed particle's subtree beyond `budget` particles is left to the
+    // next launch, which spreads the stack's entries over all of its waves:
+    // one large subtree no longer holds a single wave while the others idle
+    if (TPm.budget > 0 && walked >= TPm.budget && st.sp > 0) {
+      unsigned long long o = 0;
+      if (lane == 0) o = atomicAdd(TPm.n_spill_out, (unsigned long long)st.sp);
+      o = shfl_u64(o, 0);
+      if (lane < st.sp && (int64_t)(o + lane) < TPm.spill_cap) TPm.spill_out[o + lane] = st.lo;
+      if (lane + 64 < st.sp && (int64_t)(o + lane + 64) < TPm.spill_cap) TPm.spill_out[o + lane + 64] = st.hi;
+      const int64_t past = (int64_t)o + st.sp - TPm.spill_cap;
+      if (past > 0) n_lost += (unsigned long long)(past < st.sp ? past : st.sp);  // reported as an internal error
+      st.sp = 0;
+    }
     int64_t slot;
     int64_t rr = -1;  // the claimed item (-1: popped from the stack)
     if (st.sp > 0) {
       slot = st.pop();
     } else {
+      walked = 0;
       unsigned long long c = 0;
       if (lane == 0) c = atomicAdd(counters + C_TRANSPORT, 1ull);
       rr = (int64_t)shfl_u64(c, 0);
@@ -140,6 +156,7 @@ __global__ __launch_bounds__(256) void k_tail(DevTables T, DevStore S, uint64_t 
       else if (rr < nres + n) slot = first + (rr - nres);
       else slot = TPm.spill_in[rr - nres - n];
     }
+    ++walked;
     const bool resumed = rr >= 0 && rr < nres;
     // ---- the particle: constants from the store, the start state from the
     // store or the resume record (k_transport's refill)

@@ -94,5 +94,5 @@ def test_tuning_dict_round_trip():
     d = {"batch_lanes": 40, "yield_steps": 32, "transport_waves": 16, "emit_waves": 24,
          "draw_waves": [12, 2, 8, 2, 2, 2, 2, 2], "min_group_lanes": 4, "sweep_group": 8, "group_requests": 2, "small_sweep": 4096, "small_yield_steps": 80, "graph": 1,
          "sweep_bound": 1 << 22, "resume_cap": 1 << 19,
-         "tail_sweep": 2048, "tail_waves": 4, "tail_depth": 16}
+         "tail_sweep": 2048, "tail_waves": 4, "tail_depth": 16, "tail_budget": 32}
     assert _lib.Tuning.from_dict(d).as_dict() == d

@@ -11,7 +11,7 @@ from petite_amd import autotune
 POINT = {"batch_lanes": 40, "yield_steps": 32, "transport_waves": 16, "emit_waves": 24,
          "draw_waves": [12, 2, 8, 2, 2, 2, 2, 2], "min_group_lanes": 4, "sweep_group": 16, "group_requests": 4, "small_sweep": 16384, "small_yield_steps": 40, "graph": 2,
          "sweep_bound": 1 << 23, "resume_cap": 1 << 20,
-         "tail_sweep": 4096, "tail_waves": 8, "tail_depth": 128}
+         "tail_sweep": 16384, "tail_waves": 8, "tail_depth": 128, "tail_budget": 16}
 
 
 def test_save_and_load_round_trip(tmp_path):

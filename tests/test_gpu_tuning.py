@@ -10,7 +10,8 @@ batches, one-wave launches and single-lane draw groups.  The defaults walk
 these 48 showers depth-first after the first host check (k_tail); TINY and
 WIDE never do, so they hold the walk to the sweeps' records; DEEP walks from
 the second sweep with a one-entry stack, so most children go through the
-spill lists and several launches, hand-ons of the first sweep included.
+spill lists and several launches, hand-ons of the first sweep included;
+BUDGET leaves every subtree past 3 particles per claim to later launches.
 """
 import math
 
@@ -39,6 +40,9 @@ WIDE = {"batch_lanes": 64, "yield_steps": 1 << 30, "transport_waves": 8, "emit_w
         "draw_waves": [4, 1, 3, 1, 1, 1, 1, 1], "min_group_lanes": 64, "sweep_group": 1, "small_sweep": 1 << 30,
         "small_yield_steps": 2, "tail_sweep": 1}
 DEEP = {"yield_steps": 2, "sweep_group": 1, "small_sweep": 1, "tail_sweep": 1 << 30, "tail_waves": 1, "tail_depth": 1}
+# the walk from the first host check with a budget of 3 particles per claim:
+# every larger subtree continues in later launches
+BUDGET = {"sweep_group": 2, "tail_sweep": 1 << 30, "tail_budget": 3}
 
 
 def _primaries(n, seed):
@@ -78,7 +82,7 @@ def reference_run(lead):
     return prim, batch
 
 
-@pytest.mark.parametrize("name,tuning", [("tiny", TINY), ("wide", WIDE), ("deep", DEEP)])
+@pytest.mark.parametrize("name,tuning", [("tiny", TINY), ("wide", WIDE), ("deep", DEEP), ("budget", BUDGET)])
 def test_records_do_not_depend_on_tuning(lead, reference_run, name, tuning):
     prim, ref = reference_run
     ctx = lead.device_context()
@@ -101,7 +105,8 @@ def test_records_do_not_depend_on_tuning(lead, reference_run, name, tuning):
     if name in ("tiny", "wide"):
         assert batch.stats["launches_tail"] == 0
     else:
-        # one-entry stacks: the spilled children take further launches
+        # one-entry stacks or a 3-particle budget: the spilled children take
+        # further launches
         assert batch.stats["launches_tail"] > 1
     assert ref.stats["launches_tail"] >= 1
     assert batch.stats["n_steps"] == ref.stats["n_steps"]
@@ -151,4 +156,6 @@ def test_tuning_validation(lead):
     d = ctx.set_tuning(None)
     assert d["batch_lanes"] == 44 and d["yield_steps"] == 40 and d["sweep_bound"] == 1 << 23
     assert d["resume_cap"] == 1 << 20 and d["draw_waves"][0] == 16
-    assert d["tail_sweep"] == 4096 and d["tail_waves"] == 8 and d["tail_depth"] == 128
+    assert d["tail_sweep"] == 16384 and d["tail_waves"] == 8 and d["tail_depth"] == 128 and d["tail_budget"] == 16
+    assert ctx.set_tuning({"tail_budget": 1 << 30})["tail_budget"] == 1 << 30  # no limit
+    ctx.set_tuning(None)
