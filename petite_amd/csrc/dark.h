@@ -686,20 +686,94 @@ __device__ inline void bound_wd(const DevTables& T, const BoundView& B, int k, i
   if (GRAD) dW = W * kLn10 * dL;
 }
 
+// Record indices one yield kernel visits (the rest contribute nothing to it).
+struct RecordList {
+  const int32_t* idx;
+  const unsigned long long* count;  // on the device
+};
+
+// Species bit of a record's pid (0 e-, 1 e+, 2 photon, 3 mu-, 4 mu+; 5 other).
+__device__ inline uint32_t species_bit(int pid) {
+  switch (pid) {
+    case 11: return 1u;
+    case -11: return 2u;
+    case 22: return 4u;
+    case 13: return 8u;
+    case -13: return 16u;
+    default: return 32u;
+  }
+}
+
+// The records of each yield kernel (species in mask_a / mask_b): the scans
+// then run over the records that carry weight only (in a lead shower about
+// half of them for each kernel) instead of masking the other lanes in every
+// pass.  Each wave lists a run of kListRun x 64 consecutive records in record
+// order: a first look counts its entries, one atomic per block and list
+// reserves the block's share (atomics on one address are served one after
+// another, so one per wave and 64 records cost more than the scans save), a
+// second look writes them.
+constexpr int kListRun = 16;
+__global__ __launch_bounds__(256) void k_yield_lists(DevStore S, int64_t n, uint32_t mask_a, uint32_t mask_b,
+                                                     int32_t* list_a, int32_t* list_b, unsigned long long* counts) {
+  __shared__ unsigned long long s_cnt[2][4], s_base[2];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  constexpr int64_t kWaveRun = 64 * kListRun, kBlockRun = 4 * kWaveRun;
+  for (int64_t b0 = (int64_t)blockIdx.x * kBlockRun; b0 < n; b0 += (int64_t)gridDim.x * kBlockRun) {
+    const int64_t w0 = b0 + wid * kWaveRun;
+    unsigned long long ca = 0, cb = 0;
+    for (int j = 0; j < kListRun; ++j) {
+      const int64_t i = w0 + j * 64 + lane;
+      const uint32_t bit = i < n ? species_bit(S.pid[i]) : 0u;
+      ca += (unsigned long long)__popcll(__ballot((bit & mask_a) != 0u));
+      cb += (unsigned long long)__popcll(__ballot((bit & mask_b) != 0u));
+    }
+    if (lane == 0) {
+      s_cnt[0][wid] = ca;
+      s_cnt[1][wid] = cb;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+      const int l = threadIdx.x;
+      const unsigned long long tot = s_cnt[l][0] + s_cnt[l][1] + s_cnt[l][2] + s_cnt[l][3];
+      s_base[l] = tot ? atomicAdd(counts + l, tot) : 0ull;
+    }
+    __syncthreads();
+    unsigned long long oa = s_base[0], ob = s_base[1];
+    for (int w = 0; w < wid; ++w) {
+      oa += s_cnt[0][w];
+      ob += s_cnt[1][w];
+    }
+    for (int j = 0; j < kListRun; ++j) {
+      const int64_t i = w0 + j * 64 + lane;
+      const uint32_t bit = i < n ? species_bit(S.pid[i]) : 0u;
+      const bool in_a = (bit & mask_a) != 0u, in_b = (bit & mask_b) != 0u;
+      const unsigned long long ma = __ballot(in_a), mb = __ballot(in_b);
+      if (in_a) list_a[oa + __popcll(ma & below)] = (int32_t)i;
+      if (in_b) list_b[ob + __popcll(mb & below)] = (int32_t)i;
+      oa += (unsigned long long)__popcll(ma);
+      ob += (unsigned long long)__popcll(mb);
+    }
+    __syncthreads();  // s_cnt and s_base are read before the next run writes them
+  }
+}
+
 // One float64 pass over the record: acc[2k] += w W(m_k), acc[2k+1] += w dW/dm(m_k)
 // (GRAD = false: the yield alone).  Between trained masses j and j+1 a
 // particle's weight is interpolated geometrically, W = exp(ln w_j + t ln(w_j+1 /
 // w_j)), where both are positive, linearly otherwise.
 template <bool GRAD>
 __global__ __launch_bounds__(256) void k_dark_yield(DevTables T, const YieldTables* __restrict__ Yp, DevStore S,
-                                                    int64_t n, YieldPoints P, bool bound_on, double* acc) {
+                                                    RecordList L, YieldPoints P, bool bound_on, double* acc) {
   const YieldTables& Y = *Yp;
   double y[kYieldBatch], dy[kYieldBatch];
 #pragma unroll
   for (int k = 0; k < kYieldBatch; ++k) y[k] = dy[k] = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const PetiteDarkDesc& d0 = Y.d[0];
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  const int64_t n = (int64_t)*L.count;
+  for (int64_t ii = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; ii < n; ii += stride) {
+    const int64_t i = L.idx[ii];
     const int pid = S.pid[i];
     const double E0 = (double)S.p0[i].x;
     const double w = (double)S.r0[i].w;
@@ -777,13 +851,15 @@ __global__ __launch_bounds__(256) void k_dark_yield(DevTables T, const YieldTabl
 // accumulators.  A kernel of its own keeps the interpolation kernel's
 // registers (its three table lookups per mass would double them).
 template <bool GRAD>
-__global__ __launch_bounds__(256) void k_bound_yield(DevTables T, DevStore S, int64_t n, int32_t n_pts, int32_t ann,
+__global__ __launch_bounds__(256) void k_bound_yield(DevTables T, DevStore S, RecordList L, int32_t n_pts, int32_t ann,
                                                      int32_t comp, double ln_eps2, BoundView Bv, double* acc) {
   double y[kYieldBatch], dy[kYieldBatch];
 #pragma unroll
   for (int k = 0; k < kYieldBatch; ++k) y[k] = dy[k] = 0.0;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+  const int64_t n = (int64_t)*L.count;
+  for (int64_t ii = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; ii < n; ii += stride) {
+    const int64_t i = L.idx[ii];
     const int pid = S.pid[i];
     const bool is_ann = ann && pid == -11, is_comp = comp && pid == 22;
     if (!is_ann && !is_comp) continue;
