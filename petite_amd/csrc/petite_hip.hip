@@ -134,6 +134,28 @@ __device__ inline void wave_stat(unsigned long long* ctr, int idx, unsigned long
   if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr + idx, v);
 }
 
+// Adds v[k], summed over the block, to *dst[k]: one atomic per block and
+// counter.  Every thread of the block calls it (it synchronises the block).
+// Counters that every wave of a kernel updates when it ends would otherwise
+// take one atomic per wave, and atomics on one address are served one after
+// another: at the end of a large draw launch, 4,096 waves of them.
+template <int N>
+__device__ inline void block_stat_add(const unsigned long long (&v)[N], unsigned long long* const (&dst)[N]) {
+  __shared__ unsigned long long s_part[N][16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    const unsigned long long w = wave_sum(v[k]);
+    if (lane == 0) s_part[k][wid] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    unsigned long long t = 0;
+    for (int w = 0; w < nw; ++w) t += s_part[threadIdx.x][w];
+    if (t) atomicAdd(dst[threadIdx.x], t);
+  }
+}
+
 // Hands out request indices to the lanes in ``need`` from wave-private
 // chunks of CH consecutive indices (one atomic per chunk).
 template <int CH>
@@ -331,13 +353,14 @@ __device__ void draw_loop(const DevTables& T, const DrawLaunch& io, uint64_t see
       }
     }
   }
-  wave_stat(io.counters, C_STATS + ST_TRIALS, n_trials);
-  wave_stat(io.counters, C_STATS + ST_DRAWS, n_draw);
-  wave_stat(io.counters, C_STATS + ST_VIOL, n_viol);
-  wave_stat(io.counters, C_STATS + ST_NOSAMPLE, n_fail);
-  wave_stat(io.counters, C_PTRIALS + p, n_trials);
-  wave_stat(io.counters, C_PDRAWS + p, n_draw);
-  wave_stat(io.counters, C_PVIOL + p, n_viol);
+  // every block of a draw launch calls draw_loop (a block-uniform choice), so
+  // the whole block reaches this point
+  unsigned long long* const ctr = io.counters;
+  const unsigned long long v[7] = {n_trials, n_draw, n_viol, n_fail, n_trials, n_draw, n_viol};
+  unsigned long long* const dst[7] = {ctr + C_STATS + ST_TRIALS, ctr + C_STATS + ST_DRAWS, ctr + C_STATS + ST_VIOL,
+                                      ctr + C_STATS + ST_NOSAMPLE, ctr + C_PTRIALS + p, ctr + C_PDRAWS + p,
+                                      ctr + C_PVIOL + p};
+  block_stat_add<7>(v, dst);
 }
 
 // One kernel per process: the integrand, the map dimension and the
@@ -1281,7 +1304,11 @@ __global__ __launch_bounds__(64 * kEmitWaves) void k_emit(DevTables T, DevStore 
     }
   }
   }
-  if (stats) wave_stat(stats, ST_DROPPED, n_dropped);
+  if (stats) {
+    const unsigned long long v[1] = {n_dropped};
+    unsigned long long* const dst[1] = {stats + ST_DROPPED};
+    block_stat_add<1>(v, dst);
+  }
 }
 
 // ------------------------------------------------------- single vertices --
