@@ -34,6 +34,10 @@ SPACE = {
     "emit_waves": [16, 24, 32, 40],
     "draw_waves.0": [8, 12, 16, 20],
     "draw_waves.2": [4, 6, 8, 12],
+    "draw_waves.1": [1, 2, 4, 8],
+    "draw_waves.3": [1, 2, 4],
+    "draw_waves.4": [1, 2, 4],
+    "draw_waves.5": [1, 2, 4],
     "min_group_lanes": [2, 4, 8],
     "group_requests": [1, 2, 4],
     "sweep_group": [8, 16, 32],
