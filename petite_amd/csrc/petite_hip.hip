@@ -2242,7 +2242,7 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   double t_tail = 0.0;
   const unsigned tail_blocks = blocks_for(c, tn.tail_waves);
   if (c->timing) {
-    while ((int)c->tev.size() < 4 * group + 2) {
+    while ((int)c->tev.size() < 4 * group + 2 * kTailGroup) {
       hipEvent_t e;
       HIP_TRY(hipEventCreate(&e));
       c->tev.push_back(e);
@@ -2300,28 +2300,35 @@ int petite_run_showers(PetiteCtx* c, const PetitePrimary* prim, int64_t n, uint6
   while (true) {
     if (tail) {
       if (!c->tail_spill) HIP_TRY(hipMalloc(&c->tail_spill, 2 * kTailSpill * sizeof(int32_t)));
-      hipLaunchKernelGGL(k_tail_setup, dim3(1), dim3(64), 0, s, ctr, c->drange, store->capacity, rcap, tpar);
-      HIP_TRY(hipGetLastError());
-      if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * group], s));
-      const TailParams tp{c->tail_spill + (int64_t)tpar * kTailSpill, ctr + C_SPILL + tpar,
-                          c->tail_spill + (int64_t)(tpar ^ 1) * kTailSpill, ctr + C_SPILL + (tpar ^ 1), kTailSpill,
-                          tn.tail_depth, tn.tail_budget};
-      hipLaunchKernelGGL(k_tail, dim3(tail_blocks), dim3(256), lds, s, c->dev, S, seed, shower_energy, ctr,
-                         c->dsig_lds, (const int64_t*)c->drange, ResumeBuf{c->dresume, rcap}, tp);
-      HIP_TRY(hipGetLastError());
-      if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * group + 1], s));
-      hipLaunchKernelGGL(k_tail_finish, dim3(1), dim3(64), 0, s, ctr, c->drange, store->capacity);
-      HIP_TRY(hipGetLastError());
-      tpar ^= 1;
-      ++tail_launches;
+      // the first walk alone (it often leaves no spill), then kTailGroup per
+      // host check: a launch whose inputs are empty ends at once
+      const int ng = tail_launches == 0 ? 1 : kTailGroup;
+      for (int g = 0; g < ng; ++g) {
+        hipLaunchKernelGGL(k_tail_setup, dim3(1), dim3(64), 0, s, ctr, c->drange, store->capacity, rcap, tpar);
+        HIP_TRY(hipGetLastError());
+        if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * group + 2 * g], s));
+        const TailParams tp{c->tail_spill + (int64_t)tpar * kTailSpill, ctr + C_SPILL + tpar,
+                            c->tail_spill + (int64_t)(tpar ^ 1) * kTailSpill, ctr + C_SPILL + (tpar ^ 1), kTailSpill,
+                            tn.tail_depth, tn.tail_budget};
+        hipLaunchKernelGGL(k_tail, dim3(tail_blocks), dim3(256), lds, s, c->dev, S, seed, shower_energy, ctr,
+                           c->dsig_lds, (const int64_t*)c->drange, ResumeBuf{c->dresume, rcap}, tp);
+        HIP_TRY(hipGetLastError());
+        if (c->timing) HIP_TRY(hipEventRecord(c->tev[4 * group + 2 * g + 1], s));
+        hipLaunchKernelGGL(k_tail_finish, dim3(1), dim3(64), 0, s, ctr, c->drange, store->capacity);
+        HIP_TRY(hipGetLastError());
+        tpar ^= 1;
+        ++tail_launches;
+      }
       unsigned long long h_spill = 0;
       HIP_TRY(hipMemcpyAsync(&h_store, ctr + C_STORE, sizeof(h_store), hipMemcpyDeviceToHost, s));
       HIP_TRY(hipMemcpyAsync(&h_spill, ctr + C_SPILL + tpar, sizeof(h_spill), hipMemcpyDeviceToHost, s));
       HIP_TRY(hipStreamSynchronize(s));
       if (c->timing) {
-        float ms = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&ms, c->tev[4 * group], c->tev[4 * group + 1]));
-        t_tail += ms;
+        for (int g = 0; g < ng; ++g) {
+          float ms = 0.0f;
+          HIP_TRY(hipEventElapsedTime(&ms, c->tev[4 * group + 2 * g], c->tev[4 * group + 2 * g + 1]));
+          t_tail += ms;
+        }
       }
       if ((int64_t)h_store > store->capacity) break;
       if (h_spill == 0) break;  // every particle walked

@@ -25,6 +25,7 @@
 #pragma once
 
 constexpr int kTailMaxDepth = 128;  // two stack entries per lane
+constexpr int kTailGroup = 8;       // walk launches queued per host check after the first
 constexpr int64_t kTailSpill = 1 << 20;  // entries per spill list
 
 struct TailParams {
