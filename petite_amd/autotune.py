@@ -42,7 +42,7 @@ SPACE = {
     "group_requests": [1, 2, 4],
     "sweep_group": [8, 16, 32],
     "small_sweep": [1, 4096, 16384, 65536, 262144],
-    "small_yield_steps": [40, 160, 1 << 30],
+    "small_yield_steps": [24, 32, 40, 160, 1 << 30],
     "graph": [1, 2],
     "tail_sweep": [1, 1024, 4096, 16384, 65536],
     "tail_waves": [4, 8, 16],
