@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -665,12 +666,18 @@ constexpr int kNprocSM = 8;
 __device__ inline int info_pid(int2 v) { return (int)(int16_t)(v.x & 0xFFFF); }
 __device__ inline int info_gen(int2 v) { return (v.x >> 16) & 0xFFFF; }
 
-// n*sigma tables in LDS.  Header per process p (kLdsHeader doubles): ln E0,
-// (n-1)/ln(E_last/E0), offset of the interval records, n, E_first, E_last and
-// the value of a one-point table.  Interval i is one 32-byte record
-// {x_i, x_i+1, y_i, (y_i+1 - y_i)/(x_i+1 - x_i)}, so a lookup is one header
-// read and one record read; the value is slope * (E - x_i) + y_i, the
-// arithmetic of scipy's interp1d (0 outside the table).
+// n*sigma tables in LDS.  Header per process p (kLdsHeader doubles):
+//   [0] float log2 E0 (low word), float nodes per log2 unit (high word; 0:
+//       no node index, binary search);
+//   [1] int offset of interval record -1 (low word), int n - 1 (high word);
+//   [2] E_first, [3] E_last (an empty range for an empty table);
+//   [4] the value of a one-point table.
+// Interval i (0 <= i <= n-2) is one 32-byte record {x_i, x_i+1, y_i,
+// (y_i+1 - y_i)/(x_i+1 - x_i)}; the value is slope * (E - x_i) + y_i, the
+// arithmetic of scipy's interp1d (0 outside the table).  Record -1 (a copy of
+// record 0) and record n-1 (a copy of record n-2) pad the interval list, so
+// the node-indexed lookup reads the two records around a node without bounds
+// checks (see LdsSigma::nsigma).
 constexpr int kLdsHeader = 8;
 
 // Channels of a charged species' n*sigma sum, in the order of its terms
@@ -686,48 +693,46 @@ __device__ inline uint32_t sum_channels(int pid) {
 
 struct LdsSigma {
   const double* base;
-  // lf = logf(E), shared by the lookups of one energy (geometric-grid guess)
-  __device__ inline double nsigma(int p, double E, float lf) const {
+  // l2 = log2f(E), shared by the lookups of one energy.  On a geometric grid
+  // the node nearest E is j = round((l2 - log2 E0) * nodes per log2 unit)
+  // (the host checks that every node lies within 0.4 of its index, and the
+  // float32 index is off by less than 1e-4, so E lies between nodes j-1 and
+  // j+1); the interval is then record j if E >= x_j, else record j-1: the
+  // largest i <= n-2 with x_i <= E, as a search finds it, from one header
+  // and one 64-byte record read and no branch.
+  __device__ inline double nsigma(int p, double E, float l2) const {
     const double* h = base + kLdsHeader * p;
-    const int n = (int)h[3];
-    if (n <= 0) return 0.0;
-    if (!(E >= h[4]) || E > h[5]) return 0.0;
-    if (n == 1) return h[6];
-    const double* iv = base + (int)h[2];
-    int lo;
-    const double inv_dlog = h[1];
-    if (inv_dlog > 0.0) {
-      lo = (int)((lf - (float)h[0]) * (float)inv_dlog);
-      lo = lo < 0 ? 0 : (lo > n - 2 ? n - 2 : lo);
+    const long long w0 = __double_as_longlong(h[0]), w1 = __double_as_longlong(h[1]);
+    const float log2E0 = __int_as_float((int)w0), per = __int_as_float((int)(w0 >> 32));
+    const double* rm1 = base + (int)w1;  // record -1
+    const int nm1 = (int)(w1 >> 32);     // n - 1
+    const bool in = (E >= h[2]) & (E <= h[3]);
+    double v;
+    if (per > 0.0f) {
+      int j = (int)((l2 - log2E0) * per + 0.5f);
+      j = min(max(j, 0), nm1);
+      const double* r = rm1 + 4 * j;
+      const double va = r[3] * (E - r[0]) + r[2];
+      const double vb = r[7] * (E - r[4]) + r[6];
+      v = E >= r[4] ? vb : va;
+    } else if (nm1 <= 0) {
+      v = nm1 == 0 ? h[4] : 0.0;  // a one-point table, or none
     } else {
-      lo = 0;
-      int hi = n - 1;
+      // a grid that is not geometric: binary search for the same interval
+      const double* iv = rm1 + 4;
+      int lo = 0, hi = nm1;
       while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
         if (iv[4 * mid] <= E) lo = mid; else hi = mid;
       }
+      const double* r = iv + 4 * lo;
+      v = r[3] * (E - r[0]) + r[2];
     }
-    // the largest lo <= n-2 with x_lo <= E (the guess is off by one at most
-    // near a node, where float32 log and the float64 grid disagree)
-    const double* r = iv + 4 * lo;
-    double xl = r[0], xh = r[1];
-    while (lo > 0 && xl > E) {
-      --lo;
-      r -= 4;
-      xl = r[0];
-      xh = r[1];
-    }
-    while (lo < n - 2 && xh <= E) {
-      ++lo;
-      r += 4;
-      xl = r[0];
-      xh = r[1];
-    }
-    return r[3] * (E - xl) + r[2];
+    return in ? v : 0.0;
   }
   // _NSigmaElectron/Photon/Positron/Muon (shower.py:355-366), terms in their order
   __device__ inline double nsigma_sum(int pid, double E) const {
-    const float lf = __logf((float)E);
+    const float lf = __log2f((float)E);
     switch (pid) {
       case 11: return nsigma(BREM, E, lf) + nsigma(MOLLER, E, lf);
       case -11: return nsigma(BREM, E, lf) + nsigma(BHABHA, E, lf) + nsigma(ANN, E, lf);
@@ -741,7 +746,7 @@ struct LdsSigma {
   // so a wave that mixes species does not run each species' sum in turn.
   __device__ inline double nsigma_sum_ch(uint32_t ch, double E) const {
     if ((ch & 0xFu) == 0xFu) return 0.0;
-    const float lf = __logf((float)E);
+    const float lf = __log2f((float)E);
     double s = nsigma((int)(ch & 0xFu), E, lf) + nsigma((int)((ch >> 4) & 0xFu), E, lf);
     const uint32_t c2 = (ch >> 8) & 0xFu;
     if (c2 != 0xFu) s += nsigma((int)c2, E, lf);
@@ -983,7 +988,7 @@ __global__ __launch_bounds__(256) void k_transport(DevTables T, DevStore S, int6
             default: break;
           }
           if (c0 >= 0) {
-            const float lf = __logf((float)E);
+            const float lf = __log2f((float)E);
             const double w0 = L.nsigma(c0, E, lf), w1 = L.nsigma(c1, E, lf);
             const double w2 = c2 >= 0 ? L.nsigma(c2, E, lf) : 0.0;
             const double SC = c2 >= 0 ? w0 + w1 + w2 : w0 + w1;
@@ -1880,27 +1885,55 @@ int petite_ctx_create(const PetiteTableDesc* desc, const double* blob64, int64_t
   T.mcs_cB = (float)(2.007e-5 * pow(Z, 2.0 / 3.0));
   T.mcs_cC = (float)(3.34 * (Z * kAlpha) * (Z * kAlpha));
   {
-    // n*sigma tables of the transport kernel, laid out for LDS
+    // n*sigma tables of the transport kernel, laid out for LDS (LdsSigma)
+    auto pack = [](uint32_t lo, uint32_t hi) {
+      const uint64_t b = (uint64_t)lo | ((uint64_t)hi << 32);
+      double d;
+      std::memcpy(&d, &b, sizeof d);
+      return d;
+    };
+    auto fbits = [](float f) {
+      uint32_t u;
+      std::memcpy(&u, &f, sizeof u);
+      return u;
+    };
     std::vector<double> lds(kLdsHeader * PETITE_NPROC, 0.0);
     for (int p = 0; p < PETITE_NPROC; ++p) {
       const PetiteSigmaDesc& sd = desc->sigma[p];
+      const int n = sd.n > 0 ? sd.n : 0;
       double* h = &lds[kLdsHeader * p];
-      h[0] = sd.lnE0;
-      h[1] = sd.inv_dlog;
-      h[3] = sd.n;
-      if (sd.n <= 0) continue;
+      h[0] = pack(0u, 0u);
+      h[1] = pack(0u, (uint32_t)(n - 1));
+      h[2] = 1.0;  // empty range
+      h[3] = 0.0;
+      if (n == 0) continue;
       const double* x = blob64 + sd.E_off;
       const double* y = blob64 + sd.v_off;
-      h[4] = x[0];
-      h[5] = x[sd.n - 1];
-      h[6] = y[0];
-      h[2] = (double)lds.size();
-      for (int i = 0; i + 1 < sd.n; ++i) {
-        lds.push_back(x[i]);
-        lds.push_back(x[i + 1]);
-        lds.push_back(y[i]);
-        lds.push_back((y[i + 1] - y[i]) / (x[i + 1] - x[i]));
+      h[2] = x[0];
+      h[3] = x[n - 1];
+      h[4] = y[0];
+      if (n == 1) continue;
+      // records -1 .. n-1: record -1 and n-1 repeat records 0 and n-2
+      const size_t rec_m1 = lds.size();
+      for (int i = -1; i <= n - 1; ++i) {
+        const int k = i < 0 ? 0 : (i > n - 2 ? n - 2 : i);
+        lds.push_back(x[k]);
+        lds.push_back(x[k + 1]);
+        lds.push_back(y[k]);
+        lds.push_back((y[k + 1] - y[k]) / (x[k + 1] - x[k]));
       }
+      // the node index of a geometric grid, kept only when every node lies
+      // within 0.4 of its index by the float32 constants the kernel uses
+      float log2E0 = 0.0f, per = 0.0f;
+      if (sd.inv_dlog > 0.0) {
+        log2E0 = (float)(sd.lnE0 / std::log(2.0));
+        per = (float)(sd.inv_dlog * std::log(2.0));
+        for (int k = 0; k < n && per > 0.0f; ++k)
+          if (!(x[k] > 0.0) || !(std::fabs((std::log2(x[k]) - (double)log2E0) * (double)per - k) < 0.4)) per = 0.0f;
+      }
+      // (the appends above may have moved the vector: index it afresh)
+      lds[kLdsHeader * p + 0] = pack(fbits(log2E0), fbits(per));
+      lds[kLdsHeader * p + 1] = pack((uint32_t)rec_m1, (uint32_t)(n - 1));
     }
     T.lds_doubles = (int32_t)lds.size();
     if (hipMalloc(&c->dsig_lds, lds.size() * sizeof(double)) != hipSuccess ||

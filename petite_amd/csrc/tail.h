@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void k_tail(DevTables T, DevStore S, uint64_t 
         default: break;
       }
       if (c0 >= 0) {
-        const float lf = __logf((float)E);
+        const float lf = __log2f((float)E);
         const double w0 = L.nsigma(c0, E, lf), w1 = L.nsigma(c1, E, lf);
         const double w2 = c2 >= 0 ? L.nsigma(c2, E, lf) : 0.0;
         const double SC = c2 >= 0 ? w0 + w1 + w2 : w0 + w1;
