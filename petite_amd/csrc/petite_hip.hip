@@ -367,8 +367,11 @@ __device__ void draw_loop(const DevTables& T, const DrawLaunch& io, uint64_t see
 // One kernel per process: the integrand, the map dimension and the
 // register allocation are specialised at compile time.  Launch p uses the
 // blocks [block_start[p], block_start[p+1]) of the plan, renumbered from 0.
+// The 4-D draws need 129-132 VGPRs unconstrained (3 waves per SIMD); held to
+// 128 (4 waves, no spills) they hide more of their map gathers' latency: the
+// Brem draw stage is 4 % faster.  Five waves (96 VGPRs) spill and are slower.
 template <int P>
-__global__ __launch_bounds__(256) void k_draw(DevTables T, DrawLaunch io, uint64_t seed) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_draw(DevTables T, DrawLaunch io, uint64_t seed) {
   if (io.dcount) {
     // a count past the region (a backstop: the transport never appends one)
     // is cut to the region; the overflow is reported by the run
@@ -462,7 +465,7 @@ __device__ void draw_block_one(const DevTables& T, const DrawLaunch& io, uint64_
 // remain, the per-process launches on their own streams cost more in
 // launches, forks and joins than the draws themselves.  Same draw loop, so
 // the same results.
-__global__ __launch_bounds__(256) void k_draw_small(DevTables T, DrawLaunch io, uint64_t seed, int per, uint32_t procs) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_draw_small(DevTables T, DrawLaunch io, uint64_t seed, int per, uint32_t procs) {
   const int p = (int)blockIdx.x / per;
   const int b = (int)blockIdx.x - p * per;
   if (p >= 8 || !((procs >> p) & 1u)) return;
