@@ -370,6 +370,8 @@ __device__ void draw_loop(const DevTables& T, const DrawLaunch& io, uint64_t see
 // The 4-D draws need 129-132 VGPRs unconstrained (3 waves per SIMD); held to
 // 128 (4 waves, no spills) they hide more of their map gathers' latency: the
 // Brem draw stage is 4 % faster.  Five waves (96 VGPRs) spill and are slower.
+// (k_draw_small, which holds every process's draw, spills at 128: it keeps
+// its 132.)
 template <int P>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_draw(DevTables T, DrawLaunch io, uint64_t seed) {
   if (io.dcount) {
@@ -465,7 +467,7 @@ __device__ void draw_block_one(const DevTables& T, const DrawLaunch& io, uint64_
 // remain, the per-process launches on their own streams cost more in
 // launches, forks and joins than the draws themselves.  Same draw loop, so
 // the same results.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_draw_small(DevTables T, DrawLaunch io, uint64_t seed, int per, uint32_t procs) {
+__global__ __launch_bounds__(256) void k_draw_small(DevTables T, DrawLaunch io, uint64_t seed, int per, uint32_t procs) {
   const int p = (int)blockIdx.x / per;
   const int b = (int)blockIdx.x - p * per;
   if (p >= 8 || !((procs >> p) & 1u)) return;
